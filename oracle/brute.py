@@ -1,0 +1,79 @@
+"""TEST INFRASTRUCTURE ONLY: independent brute-force Hamiltonian for tiny systems.
+
+Builds the dense CI matrix from second quantisation, with no code shared with
+the Slater-Condon restatement:
+
+    H = I0 + sum_{pq,s} h_pq a+_{ps} a_{qs}
+           + 1/2 sum_{pqrs,st} (pq|rs) a+_{ps} a+_{rt} a_{st} a_{qs}
+
+Spin orbitals are interleaved (k = 2p + s, s = 0 alpha, 1 beta), the
+reference's full-determinant order (chemistry/basic/determinants.h:104-150).
+A determinant is a+_{k1} a+_{k2} ... |0> with k1 < k2 < ...; a_k and a+_k
+pick up (-1)^(number of occupied spin orbitals below k).  The basis is the
+tensor product of the given alpha and beta strings, row index ia*nb + ib
+(tpb/davidson.h:171-173).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _annihilate(det: int, k: int):
+    if not (det >> k) & 1:
+        return 0, 0
+    sgn = -1 if bin(det & ((1 << k) - 1)).count("1") % 2 else 1
+    return sgn, det & ~(1 << k)
+
+
+def _create(det: int, k: int):
+    if (det >> k) & 1:
+        return 0, 0
+    sgn = -1 if bin(det & ((1 << k) - 1)).count("1") % 2 else 1
+    return sgn, det | (1 << k)
+
+
+def interleave(a: int, b: int, L: int) -> int:
+    d = 0
+    for p in range(L):
+        if (a >> p) & 1:
+            d |= 1 << (2 * p)
+        if (b >> p) & 1:
+            d |= 1 << (2 * p + 1)
+    return d
+
+
+def dense_hamiltonian(L, I0, h, eri, astr, bstr):
+    """eri: dense (L,L,L,L) chemist-notation (pq|rs)."""
+    dets = [interleave(int(a), int(b), L) for a in astr for b in bstr]
+    index = {d: i for i, d in enumerate(dets)}
+    n = len(dets)
+    H = np.zeros((n, n))
+    M = 2 * L
+    for col, ket in enumerate(dets):
+        H[col, col] += I0
+        occ = [k for k in range(M) if (ket >> k) & 1]
+        # one-body
+        for q in occ:
+            s1, d1 = _annihilate(ket, q)
+            for p in range(q % 2, M, 2):
+                s2, d2 = _create(d1, p)
+                if s2 == 0 or d2 not in index:
+                    continue
+                H[index[d2], col] += s1 * s2 * h[p // 2, q // 2]
+        # two-body: a+_p a+_r a_s a_q  (q, s annihilated first: a_q then a_s)
+        for q in occ:
+            sq, dq = _annihilate(ket, q)
+            for s in occ:
+                ss, ds = _annihilate(dq, s)
+                if ss == 0:
+                    continue
+                for r in range(s % 2, M, 2):
+                    sr, dr = _create(ds, r)
+                    if sr == 0:
+                        continue
+                    for p in range(q % 2, M, 2):
+                        sp, dp = _create(dr, p)
+                        if sp == 0 or dp not in index:
+                            continue
+                        H[index[dp], col] += 0.5 * sq * ss * sr * sp * eri[p // 2, q // 2, r // 2, s // 2]
+    return H
