@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the TPB sigma build (H * v) on one MI355X.
+
+A step is one FP64 sigma build Wb = H Wk over the whole basis of the
+workload: norb=36, n_alpha=n_beta=8, 20,000 alpha x 20,000 beta strings
+(4.0e8 determinants), seeded synthetic integrals (ERI rank 72).  Inputs are
+resident in HBM when the timed region starts.
+
+Printed JSON line:
+  value        sigma GDet/s = N / (seconds per sigma)
+  roofline     algorithmic bytes of one launch / its average duration, vs 8 TB/s:
+               B = 8 (N + nb * P_alpha) + 8 N   (read T row + one T row per alpha
+               pair; write W); hii, tables, strings and integrals are not counted
+  davidson     time-to-energy @1e-6 Ha: plain FP64 Davidson vs FP32 Davidson +
+               FP64 correction, same workload
+  cpu_baseline the CPU oracle (restatement of the reference's OpenMP sigma) on
+               a bounded sample of bra rows of the same workload
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD_DESC = {
+    "sci36": "norb=36 na=nb=8, 20000x20000 strings (4.0e8 dets), ERI rank 72",
+    "fci12": "norb=12 na=nb=4, FCI 495x495 strings (245,025 dets), ERI rank 24",
+    "sci58": "norb=58 na=nb=10, 60000x60000 strings (3.6e9 dets), ERI rank 116",
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, m, T_dev, Wb_dev, hii_dev, target_s):
+    """Time the CPU oracle's sigma on a bounded sample of bra rows; check the
+    sampled rows bit for bit against the GPU result."""
+    import numpy as np
+    from oracle import oracle as O
+    t0 = time.time()
+    S = O.System.from_workload(w)
+    log(f"cpu: oracle tables built in {time.time() - t0:.1f}s")
+    nb = S.nb
+    T = T_dev.cpu().numpy()
+    hii = np.zeros(S.N)
+    Wb = np.zeros(S.N)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rows = max(threads, 8)
+    S.hdiag_rows(hii, 0, rows)
+    t1 = time.time()
+    S.sigma_rows(hii, T, Wb, 0, rows)
+    dt = time.time() - t1
+    # scale the sample to ~target_s of CPU work
+    rows2 = int(min(S.na, max(rows, rows * target_s / max(dt, 1e-3))))
+    if rows2 > rows:
+        S.hdiag_rows(hii, rows, rows2)
+        Wb[:] = 0.0
+        t1 = time.time()
+        S.sigma_rows(hii, T, Wb, 0, rows2)
+        dt = time.time() - t1
+        rows = rows2
+    gpu_rows = Wb_dev[: rows * nb].cpu().numpy()
+    gpu_hii = hii_dev[: rows * nb].cpu().numpy()
+    exact = bool(np.array_equal(gpu_rows, Wb[: rows * nb]) and
+                 np.array_equal(gpu_hii, hii[: rows * nb]))
+    dets = rows * nb
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                 if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    return {"value": dets / dt / 1e9, "unit": "GDet/s", "cores": threads, "kind": "port",
+            "sample": f"bra rows 0..{rows - 1} x {nb} beta strings ({dets:.3e} dets) of the "
+                      f"same workload, one FP64 sigma, {dt:.1f}s, OpenMP, {model}",
+            "sample_bitexact_vs_gpu": exact}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="sci36")
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-davidson", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run only a plain 2 GiB device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has one GPU, so --gpus must be 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import sbd_amd
+    from sbd_amd import synth
+
+    torch.cuda.set_device(0)
+    if args.copy_calib:
+        n = (2 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n}))
+        return
+
+    t0 = time.time()
+    w = synth.make_workload(args.workload, args.scale)
+    log(f"workload {args.workload} x{args.scale}: na={len(w['a'])} nb={len(w['b'])} "
+        f"generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
+    torch.cuda.synchronize()
+    t_tables = time.time() - t0
+    counts = m.table_counts()
+    N = m.N
+    log(f"tables {counts.tolist()} in {t_tables:.2f}s; N={N:.3e}")
+    t0 = time.time()
+    hii = m.hdiag()
+    torch.cuda.synchronize()
+    t_hdiag = time.time() - t0
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1234)
+    T = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    Wb = torch.empty_like(T)
+    for _ in range(args.warmup):
+        m.run(hii, T, Wb, accumulate=False)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.time()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        m.run(hii, T, Wb, accumulate=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.time() - wall0
+    ms = ev0.elapsed_time(ev1) / args.steps
+    gdet = N / (ms * 1e-3) / 1e9
+    P_alpha = int(counts[0] + counts[1])
+    s = 8
+    B = s * (N + m.nb * P_alpha) + s * N
+    achieved = B / (ms * 1e-3) / 1e9
+    log(f"sigma {ms:.3f} ms/launch = {gdet:.2f} GDet/s, {achieved:.0f} GB/s algorithmic "
+        f"({achieved / HBM_PEAK_GBS:.1%} of peak); wall {wall:.2f}s")
+
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"sigma_{args.workload}_traffic.json")
+    if os.path.exists(prof) and args.scale == 1.0:
+        try:
+            traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": "sigma GDet/s + time-to-energy @1e-6 Ha; served solves/s with p50/p99 latency",
+        "value": round(gdet, 4), "unit": "GDet/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded splitmix64 integrals and strings)",
+        "config": {"workload": f"{args.workload}: {WORKLOAD_DESC.get(args.workload, '')}"
+                               + (f" scaled x{args.scale}" if args.scale != 1.0 else ""),
+                   "norb": w["norb"], "n_alpha_strings": m.na, "n_beta_strings": m.nb,
+                   "determinants": N, "pairs_alpha": P_alpha,
+                   "pairs_beta": int(counts[2] + counts[3]),
+                   "step": "one FP64 sigma build over all determinants",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes_per_launch": B,
+                     "kernel": "k_sigma_rows<double,double,true>"},
+        "setup_s": {"tables": round(t_tables, 3), "hdiag": round(t_hdiag, 3)},
+    }
+
+    if not args.no_davidson:
+        res = {}
+        for label, prec in (("fp64", 0), ("mixed", 1)):
+            opt = sbd_amd.SBD(max_it=10, max_nb=10, eps=1e-7, precision=prec, fp32_eps=1e-4)
+            t0 = time.time()
+            E, _, rep, hist = m.diag(opt, history=True)
+            res[label] = {"energy": E, "steps": rep["steps"], "steps_fp32": rep["steps_fp32"],
+                          "converged": rep["converged"], "residual": rep["residual"],
+                          "t_davidson": rep["t_davidson"], "wall": time.time() - t0,
+                          "hist": hist}
+            log(f"davidson {label}: E={E:.10f} steps={rep['steps']} "
+                f"t={rep['t_davidson']:.2f}s conv={rep['converged']}")
+        Eref = res["fp64"]["energy"]
+        dav = {}
+        for label, r in res.items():
+            tte = None
+            for E0, _, t, p in r["hist"]:
+                if p == 64 and abs(E0 - Eref) <= 1e-6:
+                    tte = t
+                    break
+            dav[label] = {"time_to_energy_s": None if tte is None else round(tte, 4),
+                          "energy": r["energy"], "abs_err_vs_fp64": abs(r["energy"] - Eref),
+                          "steps": r["steps"], "steps_fp32": r["steps_fp32"],
+                          "converged": bool(r["converged"]), "residual": r["residual"],
+                          "t_davidson_s": round(r["t_davidson"], 4)}
+        t64, tmx = dav["fp64"]["time_to_energy_s"], dav["mixed"]["time_to_energy_s"]
+        dav["speedup_mixed_vs_fp64"] = (round(t64 / tmx, 3) if t64 and tmx else None)
+        dav["eps"] = 1e-7
+        out["davidson"] = dav
+
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        m.run(hii, T, Wb, accumulate=False)
+        torch.cuda.synchronize()
+        out["cpu_baseline"] = cpu_baseline(w, m, T, Wb, hii, args.cpu_seconds)
+        log(f"cpu baseline {out['cpu_baseline']['value']:.4f} GDet/s "
+            f"(bit-exact rows: {out['cpu_baseline']['sample_bitexact_vs_gpu']})")
+    else:
+        out["cpu_baseline"] = None
+    m.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

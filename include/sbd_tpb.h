@@ -1,0 +1,184 @@
+/*
+ * sbd_tpb.h -- C ABI of the MI355X tensor-product-basis (TPB) selected-basis
+ * diagonalisation library (libsbd_tpb.so).
+ *
+ * It replaces the reference's TPB hot path behind its own host API and
+ * operator plug-in interface (paths relative to the reference repository,
+ * include/sbd/...):
+ *
+ *   reference                                            here
+ *   -------------------------------------------------   ---------------------------
+ *   sbd::SetupIntegrals  chemistry/basic/makeintegrals.h:124-174
+ *                                                        sbd_ctx_create_fcidump
+ *   sbd::MakeHelpers     chemistry/tpb/helper.h:729-873 (+GenerateExcitation :185-342)
+ *                                                        sbd_make_helpers, sbd_table_*
+ *   sbd::makeQChamDiagTerms chemistry/tpb/qcham.h:290-340
+ *                                                        sbd_hdiag
+ *   MultBase<ElemT>::run chemistry/basic/mult.h:94-102 / MultTPBThrust::run
+ *                        chemistry/tpb/mult_thrust.h:88-90, 1170-1580;
+ *                        CPU sbd::mult chemistry/tpb/mult.h:625-881
+ *                                                        sbd_sigma (accumulate = 1)
+ *   sbd::Davidson        chemistry/tpb/davidson.h:707-970,
+ *                        chemistry/basic/davidson_thrust.h:97-104
+ *                                                        sbd_davidson
+ *   sbd::tpb::diag       chemistry/tpb/sbdiag.h:151-526 (method 0)
+ *                                                        sbd_diag
+ *   sbd::InnerProduct    framework/dm_vector.h:23-51     sbd_inner
+ *
+ * Conventions
+ *   - Strings are packed one uint64 per alpha/beta string (orbital p -> bit p);
+ *     norb <= 64.  sbd_pack_strings converts the reference's det_vector words
+ *     (bit_length bits per size_t, little-endian) to this form.  Lists must be
+ *     sorted ascending and unique, as LoadAlphaDets/sort_bitarray leave them.
+ *   - Vectors are indexed W[ia * nb + ib] (beta fastest), as the reference's
+ *     one-rank layout (tpb/davidson.h:171-173).
+ *   - Functions taking vectors take DEVICE pointers and a HIP stream
+ *     (hipStream_t passed as void*, NULL = the null stream).
+ *   - Every function returns an sbd_status; sbd_last_error(ctx) gives text.
+ *     The library never calls exit() and never throws across the ABI.
+ *   - A context is used by one host thread at a time; distinct contexts may be
+ *     used concurrently.
+ */
+#ifndef SBD_TPB_H
+#define SBD_TPB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SBD_OK = 0,
+  SBD_ERR_ARG = 1,      /* invalid argument (reference: std::invalid_argument) */
+  SBD_ERR_IO = 2,       /* unreadable file (reference: std::runtime_error) */
+  SBD_ERR_DEVICE = 3,   /* HIP error (reference: exit(EXIT_FAILURE)) */
+  SBD_ERR_MEMORY = 4,   /* device allocation failed */
+  SBD_ERR_STATE = 5,    /* call out of order (e.g. sigma before helpers) */
+  SBD_ERR_NOT_CONVERGED = 6
+} sbd_status;
+
+typedef struct sbd_ctx sbd_ctx;
+
+/* Options of sbd::tpb::SBD (tpb/sbdiag.h:24-54) that apply to one rank, plus
+ * the fields this library adds (precision, verbose).  sbd_options_default
+ * fills the reference's defaults. */
+typedef struct {
+  int method;        /* 0 = Davidson (the only method here)                  */
+  int max_it;        /* restarts; default 1                                    */
+  int max_nb;        /* subspace size; default 10                              */
+  double eps;        /* residual-norm tolerance; default 1e-4                  */
+  double max_time;   /* seconds; default 86400                                 */
+  int init;          /* 0 = Hartree-Fock determinant (W[0] = 1); 1 = random    */
+  uint64_t seed;     /* default 1729 (used by init = 1)                        */
+  int precision;     /* 0 = FP64; 1 = FP32 vectors/sigma + FP64 reductions,
+                        Ritz and Gram-Schmidt, then an FP64 correction sweep   */
+  double fp32_eps;   /* precision = 1: residual at which FP32 hands over to FP64
+                        (default 1e-4)                                         */
+  int verbose;       /* 1 = print per-step lines like the reference            */
+} sbd_options;
+
+/* Per-solve report. */
+typedef struct {
+  int steps;              /* Davidson steps taken (sigma builds inside Davidson) */
+  int converged;          /* 1 if the residual fell below eps                    */
+  double residual;        /* last residual norm                                  */
+  double energy;          /* Ritz value of the last step                         */
+  double t_tables;        /* seconds: excitation tables                          */
+  double t_hdiag;         /* seconds: diagonal                                   */
+  double t_davidson;      /* seconds: Davidson                                   */
+  double t_sigma;         /* seconds inside sigma kernels during Davidson        */
+  int steps_fp32;         /* precision = 1: steps taken in FP32                  */
+} sbd_report;
+
+const char* sbd_status_string(int status);
+const char* sbd_last_error(const sbd_ctx* ctx);
+void sbd_options_default(sbd_options* opt);
+
+/* Packs n strings given as the reference's det_vector rows (words_per_string
+ * = ceil(norb / bit_length) size_t words, bit_length bits each; replaces
+ * from_string / det_vector storage, framework/bit_manipulation.h:219-235)
+ * into one uint64 each. */
+int sbd_pack_strings(const size_t* words, size_t n, size_t bit_length, int norb, uint64_t* out);
+
+/* Parses a text FCIDUMP exactly as sbd::LoadFCIDump (framework/fcidump.h:29-95)
+ * and returns NORB, NELEC and the entry count.  Entries are read with
+ * sbd_fcidump_entries into caller arrays (values[n], idx[4n]). */
+int sbd_fcidump_read(const char* path, int* norb, int* nelec, size_t* n_entries);
+int sbd_fcidump_entries(const char* path, double* values, int* idx, size_t n_entries);
+
+/* Creates a context on the current HIP device from FCIDUMP entries (applied in
+ * order, as SetupIntegrals does) and packed alpha/beta strings. */
+sbd_ctx* sbd_ctx_create_fcidump(int norb, size_t n_entries, const double* values, const int* idx,
+                                const uint64_t* a_strings, size_t na, const uint64_t* b_strings,
+                                size_t nb, int* status);
+/* Same, from I0, the L x L one-body matrix h (row-major) and the packed ERI
+ * (size B(B+1)/2, B = L(L+1)/2, chemistry/basic/integrals.h:67-110). */
+sbd_ctx* sbd_ctx_create(int norb, double I0, const double* h1, const double* eri_packed,
+                        const uint64_t* a_strings, size_t na, const uint64_t* b_strings,
+                        size_t nb, int* status);
+void sbd_ctx_destroy(sbd_ctx* ctx);
+
+/* Dimensions: out[0] = na, out[1] = nb, out[2] = N = na*nb, out[3] = norb. */
+int sbd_ctx_dims(const sbd_ctx* ctx, uint64_t* out);
+
+/* Builds the alpha and beta single/double excitation tables on the GPU
+ * (MakeHelpers at one rank: all bras x all kets, kets ascending). */
+int sbd_make_helpers(sbd_ctx* ctx);
+
+/* Table sizes: out[0..3] = alpha singles, alpha doubles, beta singles, beta
+ * doubles (total pairs). */
+int sbd_table_counts(const sbd_ctx* ctx, uint64_t* out);
+
+/* Copies one table to host memory in the reference's TaskHelpers layout:
+ * which = 0 SinglesFromAlpha/SinglesAlphaCrAn, 1 DoublesFromAlpha/DoublesAlphaCrAn,
+ * 2 SinglesFromBeta/SinglesBetaCrAn, 3 DoublesFromBeta/DoublesBetaCrAn.
+ * off[n+1] (CSR by bra), ket[count], cran[2*count] (singles) or [4*count]
+ * (doubles), spin-orbital indices as helper.h:239-335 writes them. */
+int sbd_table_get(const sbd_ctx* ctx, int which, int64_t* off, int64_t* ket, int32_t* cran);
+
+/* hii[N] = <D|H|D> for every determinant (qcham.h:290-340, ZeroExcite order). */
+int sbd_hdiag(sbd_ctx* ctx, double* hii, void* stream);
+int sbd_hdiag_f32(sbd_ctx* ctx, float* hii, void* stream);
+
+/* Sigma build, MultBase::run semantics: Wb += H * Wk with the diagonal taken
+ * from hii (accumulate = 1).  accumulate = 0 computes Wb = H * Wk without
+ * reading Wb (bit-identical to accumulating onto zeros).  Deterministic: one
+ * writer per element, fixed summation order (diagonal, alpha singles, alpha
+ * doubles, beta singles, beta doubles, alpha-beta) as tpb/mult.h:688-828. */
+int sbd_sigma(sbd_ctx* ctx, const double* hii, const double* Wk, double* Wb, int accumulate,
+              void* stream);
+/* FP32 storage and FP32 per-determinant accumulation (matrix elements are
+ * formed in FP64 and rounded once). */
+int sbd_sigma_f32(sbd_ctx* ctx, const float* hii, const float* Wk, float* Wb, int accumulate,
+                  void* stream);
+
+/* FP64 inner product on device vectors (Kahan per thread, fixed-order sum). */
+int sbd_inner(sbd_ctx* ctx, const double* x, const double* y, uint64_t n, double* out,
+              void* stream);
+
+/* Single-root Davidson on device vectors (tpb/davidson.h:707-970): W is the
+ * initial vector on entry and the normalised Ritz vector on exit; *energy is
+ * the Ritz value of the last step.  hist (optional, host, 4*max_it*max_nb
+ * doubles) receives (E0, |R|, seconds, 64) per step. */
+int sbd_davidson(sbd_ctx* ctx, const double* hii, double* W, const sbd_options* opt,
+                 double* energy, double* hist, sbd_report* rep);
+
+/* sbd::tpb::diag, method 0, one rank (tpb/sbdiag.h:151-526): tables, hii,
+ * initial vector (opt->init), Davidson, then E = <W, H W> with a fresh sigma.
+ * W_out (optional, host, N doubles) receives the final vector.  hist
+ * (optional, host, 4*hist_cap doubles) receives per Davidson step
+ * (E0, |R|, seconds since the solver started, 32 or 64 = vector precision). */
+int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out, double* hist,
+             int hist_cap, sbd_report* rep);
+
+/* Device bytes sbd_diag would allocate for these options (vectors + tables +
+ * integrals), for admission control. */
+int sbd_diag_footprint(const sbd_ctx* ctx, const sbd_options* opt, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBD_TPB_H */

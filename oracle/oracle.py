@@ -54,6 +54,8 @@ def lib():
         L.orc_syev.argtypes = [C.c_int, dp, C.c_int, dp]
         L.orc_davidson.restype = C.c_int
         L.orc_davidson.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, dp]
+        L.orc_hdiag_rows.argtypes = [C.c_void_p, dp, C.c_long, C.c_long]
+        L.orc_sigma_rows.argtypes = [C.c_void_p, dp, dp, dp, C.c_long, C.c_long]
         L.orc_det_from_ab.argtypes = [up, up, C.c_long, C.c_int, up]
         L.orc_zero_excite.restype = C.c_double
         L.orc_zero_excite.argtypes = [up, C.c_int, C.c_double, dp, dp, dp]
@@ -217,6 +219,15 @@ class System:
         Wb = np.zeros(self.N) if Wb is None else np.ascontiguousarray(Wb, dtype=np.float64).copy()
         lib().orc_sigma(self._h, _p(hii, C.c_double), _p(T, C.c_double), _p(Wb, C.c_double))
         return Wb
+
+    def hdiag_rows(self, hii, ia0, ia1):
+        """Fill rows [ia0, ia1) of a full-size hii (in place)."""
+        lib().orc_hdiag_rows(self._h, _p(hii, C.c_double), ia0, ia1)
+
+    def sigma_rows(self, hii, T, Wb, ia0, ia1):
+        """Wb[rows ia0..ia1) += (H T)[rows] on full-size vectors (in place)."""
+        lib().orc_sigma_rows(self._h, _p(hii, C.c_double), _p(T, C.c_double),
+                             _p(Wb, C.c_double), ia0, ia1)
 
     def davidson(self, hii, W, max_it, nb, eps):
         W = np.ascontiguousarray(W, dtype=np.float64).copy()

@@ -340,17 +340,10 @@ void orc_sys_table(const orc_sys *S, int which, long *off, long *ket, int *cran)
 }
 
 /* makeQChamDiagTerms, chemistry/tpb/qcham.h:290-340 */
-void orc_hdiag(const orc_sys *S, double *hii) {
-  long na = S->na, nb = S->nb;
-#pragma omp parallel for schedule(static)
-  for (long ia = 0; ia < na; ia++) {
-    uint64_t D[2];
-    for (long ib = 0; ib < nb; ib++) {
-      det_from_ab(S->a[ia], S->b[ib], S->L, D);
-      hii[ia * nb + ib] = zero_excite(D, S->L, S->I0, S->I1, S->Dm, S->Xm);
-    }
-  }
-}
+void orc_hdiag_rows(const orc_sys *S, double *hii, long ia0, long ia1);
+void orc_sigma_rows(const orc_sys *S, const double *hii, const double *T, double *Wb, long ia0,
+                    long ia1);
+void orc_hdiag(const orc_sys *S, double *hii) { orc_hdiag_rows(S, hii, 0, S->na); }
 
 /*
  * mult (default, non-TRADMODE), chemistry/tpb/mult.h:625-881, one rank:
@@ -359,59 +352,7 @@ void orc_hdiag(const orc_sys *S, double *hii) {
  * row owned by one thread (round-robin ia).  Accumulates into Wb.
  */
 void orc_sigma(const orc_sys *S, const double *hii, const double *T, double *Wb) {
-  long na = S->na, nb = S->nb, N = na * nb;
-  int L = S->L, L2 = 2 * L, dsize = (2 * L + 63) / 64;
-#pragma omp parallel for
-  for (long i = 0; i < N; i++) Wb[i] += hii[i] * T[i];
-
-  for (int task_type = 2; task_type >= 0; task_type--) {
-#pragma omp parallel
-    {
-      int nth = omp_get_num_threads(), tid = omp_get_thread_num();
-      uint64_t D[2];
-      for (long ia = tid; ia < na; ia += nth) {
-        for (long ib = 0; ib < nb; ib++) {
-          long bra = ia * nb + ib;
-          det_from_ab(S->a[ia], S->b[ib], L, D);
-          if (task_type == 2) {
-            for (long o = S->as_off[ia]; o < S->as_off[ia + 1]; o++) {
-              long ket = S->as_ket[o] * nb + ib;
-              double e = one_excite(D, dsize, S->as_cr[2 * o], S->as_cr[2 * o + 1], L2, S->I1, S->I2);
-              Wb[bra] += e * T[ket];
-            }
-            for (long o = S->ad_off[ia]; o < S->ad_off[ia + 1]; o++) {
-              long ket = S->ad_ket[o] * nb + ib;
-              const int *c = S->ad_cr + 4 * o;
-              double e = two_excite(D, c[0], c[1], c[2], c[3], S->I2);
-              Wb[bra] += e * T[ket];
-            }
-          } else if (task_type == 1) {
-            for (long o = S->bs_off[ib]; o < S->bs_off[ib + 1]; o++) {
-              long ket = ia * nb + S->bs_ket[o];
-              double e = one_excite(D, dsize, S->bs_cr[2 * o], S->bs_cr[2 * o + 1], L2, S->I1, S->I2);
-              Wb[bra] += e * T[ket];
-            }
-            for (long o = S->bd_off[ib]; o < S->bd_off[ib + 1]; o++) {
-              long ket = ia * nb + S->bd_ket[o];
-              const int *c = S->bd_cr + 4 * o;
-              double e = two_excite(D, c[0], c[1], c[2], c[3], S->I2);
-              Wb[bra] += e * T[ket];
-            }
-          } else {
-            for (long o = S->as_off[ia]; o < S->as_off[ia + 1]; o++) {
-              long ja = S->as_ket[o];
-              for (long q = S->bs_off[ib]; q < S->bs_off[ib + 1]; q++) {
-                long ket = ja * nb + S->bs_ket[q];
-                double e = two_excite(D, S->as_cr[2 * o], S->bs_cr[2 * q], S->as_cr[2 * o + 1],
-                                      S->bs_cr[2 * q + 1], S->I2);
-                Wb[bra] += e * T[ket];
-              }
-            }
-          }
-        }
-      }
-    }
-  }
+  orc_sigma_rows(S, hii, T, Wb, 0, S->na);
 }
 
 /* ------------------------------------------------------------------ */
@@ -593,4 +534,72 @@ int orc_davidson(const orc_sys *S, const double *hii, double *W, int max_it, int
   free(U);
   free(E);
   return steps;
+}
+
+/* Rows [ia0, ia1) of hdiag and sigma (same arithmetic as orc_hdiag /
+ * orc_sigma); used for a bounded CPU-baseline sample of a large workload.
+ * hii, T and Wb are full-size vectors (index ia*nb+ib). */
+void orc_hdiag_rows(const orc_sys *S, double *hii, long ia0, long ia1) {
+  long nb = S->nb;
+#pragma omp parallel for schedule(static)
+  for (long ia = ia0; ia < ia1; ia++) {
+    uint64_t D[2];
+    for (long ib = 0; ib < nb; ib++) {
+      det_from_ab(S->a[ia], S->b[ib], S->L, D);
+      hii[ia * nb + ib] = zero_excite(D, S->L, S->I0, S->I1, S->Dm, S->Xm);
+    }
+  }
+}
+
+void orc_sigma_rows(const orc_sys *S, const double *hii, const double *T, double *Wb, long ia0,
+                    long ia1) {
+  long nb = S->nb;
+  int L = S->L, L2 = 2 * L, dsize = (2 * L + 63) / 64;
+#pragma omp parallel for schedule(static)
+  for (long i = ia0 * nb; i < ia1 * nb; i++) Wb[i] += hii[i] * T[i];
+  for (int task_type = 2; task_type >= 0; task_type--) {
+#pragma omp parallel
+    {
+      int nth = omp_get_num_threads(), tid = omp_get_thread_num();
+      uint64_t D[2];
+      for (long ia = ia0 + tid; ia < ia1; ia += nth) {
+        for (long ib = 0; ib < nb; ib++) {
+          long bra = ia * nb + ib;
+          det_from_ab(S->a[ia], S->b[ib], L, D);
+          if (task_type == 2) {
+            for (long o = S->as_off[ia]; o < S->as_off[ia + 1]; o++) {
+              long ket = S->as_ket[o] * nb + ib;
+              double e = one_excite(D, dsize, S->as_cr[2 * o], S->as_cr[2 * o + 1], L2, S->I1, S->I2);
+              Wb[bra] += e * T[ket];
+            }
+            for (long o = S->ad_off[ia]; o < S->ad_off[ia + 1]; o++) {
+              long ket = S->ad_ket[o] * nb + ib;
+              const int *c = S->ad_cr + 4 * o;
+              Wb[bra] += two_excite(D, c[0], c[1], c[2], c[3], S->I2) * T[ket];
+            }
+          } else if (task_type == 1) {
+            for (long o = S->bs_off[ib]; o < S->bs_off[ib + 1]; o++) {
+              long ket = ia * nb + S->bs_ket[o];
+              double e = one_excite(D, dsize, S->bs_cr[2 * o], S->bs_cr[2 * o + 1], L2, S->I1, S->I2);
+              Wb[bra] += e * T[ket];
+            }
+            for (long o = S->bd_off[ib]; o < S->bd_off[ib + 1]; o++) {
+              long ket = ia * nb + S->bd_ket[o];
+              const int *c = S->bd_cr + 4 * o;
+              Wb[bra] += two_excite(D, c[0], c[1], c[2], c[3], S->I2) * T[ket];
+            }
+          } else {
+            for (long o = S->as_off[ia]; o < S->as_off[ia + 1]; o++) {
+              long ja = S->as_ket[o];
+              for (long q = S->bs_off[ib]; q < S->bs_off[ib + 1]; q++) {
+                long ket = ja * nb + S->bs_ket[q];
+                Wb[bra] += two_excite(D, S->as_cr[2 * o], S->bs_cr[2 * q], S->as_cr[2 * o + 1],
+                                      S->bs_cr[2 * q + 1], S->I2) * T[ket];
+              }
+            }
+          }
+        }
+      }
+    }
+  }
 }

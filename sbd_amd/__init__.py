@@ -1,0 +1,341 @@
+"""sbd_amd: MI355X tensor-product-basis (TPB) Davidson and sigma build.
+
+Python host mirror of the reference's TPB operator interface, calling the HIP
+library ``libsbd_tpb.so`` through its C ABI (include/sbd_tpb.h).  Names follow
+the reference:
+
+* ``SBD``               -- options, sbd::tpb::SBD (chemistry/tpb/sbdiag.h:24-54)
+* ``LoadFCIDump``       -- framework/fcidump.h:29-95 (parsed by the library)
+* ``LoadAlphaDets``     -- chemistry/basic/makedeterminants.h:167-191
+* ``MultTPB``           -- the MultBase<ElemT> operator (chemistry/basic/mult.h:20-103):
+                           ``run(hii, Wk, Wb)`` accumulates Wb += H Wk on device
+* ``diag``              -- sbd::tpb::diag, method 0 (chemistry/tpb/sbdiag.h:151-526)
+
+Device vectors are torch tensors on the HIP device (torch is used only for
+device memory and streams).  There is no CPU fallback: if the library is
+missing, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsbd_tpb.so")
+_LIB = None
+
+EXPORTS = [
+    "sbd_status_string", "sbd_last_error", "sbd_options_default", "sbd_pack_strings",
+    "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
+    "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
+    "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
+    "sbd_diag", "sbd_diag_footprint",
+]
+
+
+class SbdError(RuntimeError):
+    pass
+
+
+class Options(C.Structure):
+    _fields_ = [("method", C.c_int), ("max_it", C.c_int), ("max_nb", C.c_int),
+                ("eps", C.c_double), ("max_time", C.c_double), ("init", C.c_int),
+                ("seed", C.c_uint64), ("precision", C.c_int), ("fp32_eps", C.c_double),
+                ("verbose", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
+                ("energy", C.c_double), ("t_tables", C.c_double), ("t_hdiag", C.c_double),
+                ("t_davidson", C.c_double), ("t_sigma", C.c_double), ("steps_fp32", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    """Load libsbd_tpb.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise SbdError(f"{LIB_PATH} not built: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, fp, up, ip, szp = (C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float),
+                               C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_size_t))
+    L.sbd_status_string.restype = C.c_char_p
+    L.sbd_status_string.argtypes = [C.c_int]
+    L.sbd_last_error.restype = C.c_char_p
+    L.sbd_last_error.argtypes = [vp]
+    L.sbd_options_default.argtypes = [C.POINTER(Options)]
+    L.sbd_pack_strings.argtypes = [szp, C.c_size_t, C.c_size_t, C.c_int, up]
+    L.sbd_fcidump_read.argtypes = [C.c_char_p, ip, ip, szp]
+    L.sbd_fcidump_entries.argtypes = [C.c_char_p, dp, ip, C.c_size_t]
+    L.sbd_ctx_create_fcidump.restype = vp
+    L.sbd_ctx_create_fcidump.argtypes = [C.c_int, C.c_size_t, dp, ip, up, C.c_size_t, up,
+                                         C.c_size_t, ip]
+    L.sbd_ctx_create.restype = vp
+    L.sbd_ctx_create.argtypes = [C.c_int, C.c_double, dp, dp, up, C.c_size_t, up, C.c_size_t, ip]
+    L.sbd_ctx_destroy.argtypes = [vp]
+    L.sbd_ctx_dims.argtypes = [vp, up]
+    L.sbd_make_helpers.argtypes = [vp]
+    L.sbd_table_counts.argtypes = [vp, up]
+    L.sbd_table_get.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int32)]
+    L.sbd_hdiag.argtypes = [vp, vp, vp]
+    L.sbd_hdiag_f32.argtypes = [vp, vp, vp]
+    L.sbd_sigma.argtypes = [vp, vp, vp, vp, C.c_int, vp]
+    L.sbd_sigma_f32.argtypes = [vp, vp, vp, vp, C.c_int, vp]
+    L.sbd_inner.argtypes = [vp, vp, vp, C.c_uint64, dp, vp]
+    L.sbd_davidson.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
+    L.sbd_diag.argtypes = [vp, C.POINTER(Options), dp, dp, dp, C.c_int, C.POINTER(Report)]
+    L.sbd_diag_footprint.argtypes = [vp, C.POINTER(Options), up]
+    _LIB = L
+    return L
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _dptr(t):
+    """Device pointer of a torch tensor (must be contiguous, on the GPU)."""
+    if not t.is_cuda or not t.is_contiguous():
+        raise SbdError("expected a contiguous device tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@dataclass
+class SBD:
+    """sbd::tpb::SBD (chemistry/tpb/sbdiag.h:24-54), one-rank fields."""
+    method: int = 0
+    max_it: int = 1
+    max_nb: int = 10
+    eps: float = 1.0e-4
+    max_time: float = 86400.0
+    init: int = 0
+    seed: int = 1729
+    bit_length: int = 20
+    precision: int = 0           # added: 1 = FP32 vectors + FP64 correction
+    fp32_eps: float = 1.0e-4     # added: FP32 -> FP64 hand-over residual
+    verbose: int = 0
+
+    def options(self) -> Options:
+        o = Options()
+        lib().sbd_options_default(C.byref(o))
+        for k in ("method", "max_it", "max_nb", "eps", "max_time", "init", "seed", "precision",
+                  "fp32_eps", "verbose"):
+            setattr(o, k, getattr(self, k))
+        return o
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        msg = lib().sbd_status_string(rc).decode()
+        if ctx is not None:
+            detail = lib().sbd_last_error(ctx).decode()
+            if detail:
+                msg += ": " + detail
+        raise SbdError(msg)
+
+
+@dataclass
+class FCIDump:
+    """sbd::FCIDump: header key/values and (value, i, j, k, l) entries in file order."""
+    header: dict = field(default_factory=dict)
+    values: np.ndarray = None
+    idx: np.ndarray = None
+
+    @property
+    def norb(self):
+        return int(self.header["NORB"])
+
+    @property
+    def nelec(self):
+        return int(self.header.get("NELEC", 0))
+
+
+def LoadFCIDump(path: str) -> FCIDump:
+    norb, nelec, n = C.c_int(), C.c_int(), C.c_size_t()
+    _check(lib().sbd_fcidump_read(path.encode(), C.byref(norb), C.byref(nelec), C.byref(n)))
+    vals = np.zeros(n.value)
+    idx = np.zeros(4 * n.value, dtype=np.int32)
+    _check(lib().sbd_fcidump_entries(path.encode(), _ptr(vals, C.c_double), _ptr(idx, C.c_int),
+                                     n.value))
+    return FCIDump({"NORB": str(norb.value), "NELEC": str(nelec.value)}, vals, idx.reshape(-1, 4))
+
+
+def from_string(s: str, L: int) -> int:
+    """framework/bit_manipulation.h:219-235: character s[L-1-i] is orbital i."""
+    v = 0
+    for i in range(L):
+        if s[L - 1 - i] == "1":
+            v |= 1 << i
+    return v
+
+
+def LoadAlphaDets(path: str, L: int) -> np.ndarray:
+    """Text string list -> sorted unique uint64 strings (makedeterminants.h:167-191)."""
+    with open(path) as f:
+        vals = {from_string(line.rstrip("\n"), L) for line in f if line.strip()}
+    return np.array(sorted(vals), dtype=np.uint64)
+
+
+def pack_strings(words: np.ndarray, bit_length: int, norb: int) -> np.ndarray:
+    """Reference det_vector rows (size_t words of bit_length bits) -> uint64 strings."""
+    words = np.ascontiguousarray(words, dtype=np.uint64)
+    n = words.shape[0]
+    out = np.zeros(n, dtype=np.uint64)
+    _check(lib().sbd_pack_strings(words.ctypes.data_as(C.POINTER(C.c_size_t)), n, bit_length, norb,
+                                  _ptr(out, C.c_uint64)))
+    return out
+
+
+class MultTPB:
+    """TPB Hamiltonian operator on one GPU (MultBase / MultTPBThrust equivalent).
+
+    Owns the device copies of strings, integrals and excitation tables."""
+
+    def __init__(self, norb, a, b, *, fcidump: FCIDump | None = None, I0=0.0, h=None, eri=None,
+                 build_tables=True):
+        self.norb = int(norb)
+        self.a = np.ascontiguousarray(a, dtype=np.uint64)
+        self.b = np.ascontiguousarray(b, dtype=np.uint64)
+        st = C.c_int(0)
+        if fcidump is not None:
+            vals = np.ascontiguousarray(fcidump.values, dtype=np.float64)
+            idx = np.ascontiguousarray(fcidump.idx, dtype=np.int32).reshape(-1)
+            h_ = lib().sbd_ctx_create_fcidump(self.norb, len(vals), _ptr(vals, C.c_double),
+                                              _ptr(idx, C.c_int), _ptr(self.a, C.c_uint64),
+                                              len(self.a), _ptr(self.b, C.c_uint64), len(self.b),
+                                              C.byref(st))
+        else:
+            h = np.ascontiguousarray(h, dtype=np.float64)
+            eri = np.ascontiguousarray(eri, dtype=np.float64)
+            h_ = lib().sbd_ctx_create(self.norb, float(I0), _ptr(h, C.c_double),
+                                      _ptr(eri, C.c_double), _ptr(self.a, C.c_uint64), len(self.a),
+                                      _ptr(self.b, C.c_uint64), len(self.b), C.byref(st))
+        if not h_:
+            _check(st.value if st.value else 1)
+        self._ctx = C.c_void_p(h_)
+        self.na, self.nb = len(self.a), len(self.b)
+        self.N = self.na * self.nb
+        if build_tables:
+            self.make_helpers()
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib().sbd_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def make_helpers(self):
+        _check(lib().sbd_make_helpers(self._ctx), self._ctx)
+
+    def table_counts(self):
+        out = np.zeros(4, dtype=np.uint64)
+        _check(lib().sbd_table_counts(self._ctx, _ptr(out, C.c_uint64)), self._ctx)
+        return out.astype(np.int64)
+
+    def table(self, which):
+        """(offsets, kets, CrAn) of table `which` in the reference layout."""
+        n = self.na if which < 2 else self.nb
+        cnt = int(self.table_counts()[which])
+        w = 2 if which % 2 == 0 else 4
+        off = np.zeros(n + 1, dtype=np.int64)
+        ket = np.zeros(max(cnt, 1), dtype=np.int64)
+        cr = np.zeros(max(w * cnt, 1), dtype=np.int32)
+        _check(lib().sbd_table_get(self._ctx, which, _ptr(off, C.c_int64), _ptr(ket, C.c_int64),
+                                   _ptr(cr, C.c_int32)), self._ctx)
+        return off, ket[:cnt], cr[:w * cnt].reshape(cnt, w)
+
+    def hdiag(self, dtype=None):
+        import torch
+        dtype = dtype or torch.float64
+        out = torch.empty(self.N, dtype=dtype, device="cuda")
+        fn = lib().sbd_hdiag if dtype == torch.float64 else lib().sbd_hdiag_f32
+        _check(fn(self._ctx, _dptr(out), _stream()), self._ctx)
+        return out
+
+    # alias matching MultTPBThrust::makeQChamDiagTerms
+    makeQChamDiagTerms = hdiag
+
+    def run(self, hii, Wk, Wb, accumulate=True):
+        """MultBase::run: Wb += H Wk (device tensors, float64 or float32)."""
+        import torch
+        if not (hii.dtype == Wk.dtype == Wb.dtype) or Wk.numel() != self.N or Wb.numel() != self.N \
+                or hii.numel() != self.N:
+            raise SbdError("run: shapes/dtypes must match the basis")
+        fn = lib().sbd_sigma if Wk.dtype == torch.float64 else lib().sbd_sigma_f32
+        _check(fn(self._ctx, _dptr(hii), _dptr(Wk), _dptr(Wb), 1 if accumulate else 0, _stream()),
+               self._ctx)
+        return Wb
+
+    def sigma(self, hii, Wk):
+        import torch
+        Wb = torch.empty_like(Wk)
+        return self.run(hii, Wk, Wb, accumulate=False)
+
+    def inner(self, x, y):
+        out = C.c_double(0.0)
+        _check(lib().sbd_inner(self._ctx, _dptr(x), _dptr(y), x.numel(), C.byref(out), _stream()),
+               self._ctx)
+        return out.value
+
+    def davidson(self, hii, W, sbd: SBD, history=False):
+        """Davidson on device vectors; W (device, float64) is updated in place."""
+        o = sbd.options()
+        E = C.c_double(0.0)
+        rep = Report()
+        hist = np.zeros(4 * sbd.max_it * sbd.max_nb)
+        _check(lib().sbd_davidson(self._ctx, _dptr(hii), _dptr(W), C.byref(o), C.byref(E),
+                                  _ptr(hist, C.c_double), C.byref(rep)), self._ctx)
+        out = (E.value, rep.as_dict())
+        if history:
+            out = out + (hist[:4 * rep.steps].reshape(-1, 4),)
+        return out
+
+    def diag(self, sbd: SBD, want_vector=False, history=False):
+        """Returns (energy, W or None, report[, history]); history rows are
+        (E0, |R|, seconds since solver start, vector precision 32/64)."""
+        o = sbd.options()
+        E = C.c_double(0.0)
+        rep = Report()
+        W = np.zeros(self.N) if want_vector else None
+        cap = sbd.max_it * sbd.max_nb * (2 if sbd.precision == 1 else 1)
+        hist = np.zeros(4 * cap)
+        _check(lib().sbd_diag(self._ctx, C.byref(o), C.byref(E),
+                              _ptr(W, C.c_double) if want_vector else None,
+                              _ptr(hist, C.c_double), cap, C.byref(rep)), self._ctx)
+        out = (E.value, W, rep.as_dict())
+        if history:
+            out = out + (hist[:4 * min(rep.steps, cap)].reshape(-1, 4),)
+        return out
+
+    def footprint(self, sbd: SBD):
+        o = sbd.options()
+        out = C.c_uint64(0)
+        _check(lib().sbd_diag_footprint(self._ctx, C.byref(o), C.byref(out)), self._ctx)
+        return out.value
+
+
+def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
+    """sbd::tpb::diag (method 0, one rank): returns (energy, W)."""
+    if loadname or savename:
+        raise SbdError("wavefunction load/save is not supported by this build")
+    m = MultTPB(fcidump.norb, adet, bdet, fcidump=fcidump)
+    try:
+        E, W, _ = m.diag(sbd_data, want_vector=True)
+    finally:
+        m.close()
+    return E, W

@@ -1,0 +1,170 @@
+// Internal definitions shared by the HIP translation units of libsbd_tpb.
+//
+// Bit conventions (reference: chemistry/basic/determinants.h:104-150):
+//   half string   : one uint64 per alpha or beta string, orbital p -> bit p (norb <= 64)
+//   determinant   : interleaved, alpha p -> spin orbital 2p, beta p -> 2p+1,
+//                   held as two uint64 words (spin orbitals 0..63, 64..127)
+//   vector layout : W[ia * nb + ib], beta index fastest (tpb/davidson.h:171-173)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sbd_tpb.h"
+
+namespace sbdx {
+
+// Excitation record: one (bra, ket) pair of strings of one spin that differ by
+// a single (2 bits) or double (4 bits) excitation.  16 bytes.
+//   ket  : ket string index
+//   code : bra-only orbitals x0 (bits 0-5), x1 (6-11); ket-only orbitals
+//          y0 (12-17), y1 (18-23), each ascending; bit 24: same-spin parity of
+//          a single excitation (number of bra bits of this spin strictly between
+//          x0 and y0, mod 2)
+//   val  : single: one-body integral h[y0][x0] (= I1.Value(a,i));
+//          double: same-spin sign x extra sign x ((AI|BJ) - (AJ|BI))
+struct alignas(16) PairRec {
+  uint32_t ket;
+  uint32_t code;
+  double val;
+};
+
+__host__ __device__ inline int code_x0(uint32_t c) { return c & 63; }
+__host__ __device__ inline int code_x1(uint32_t c) { return (c >> 6) & 63; }
+__host__ __device__ inline int code_y0(uint32_t c) { return (c >> 12) & 63; }
+__host__ __device__ inline int code_y1(uint32_t c) { return (c >> 18) & 63; }
+__host__ __device__ inline int code_par(uint32_t c) { return (c >> 24) & 1; }
+
+// bits [lo, hi) of a 64-bit word, 0 <= lo <= hi <= 64
+__host__ __device__ inline uint64_t bits_range(int lo, int hi) {
+  uint64_t h = hi >= 64 ? ~0ULL : ((1ULL << hi) - 1ULL);
+  uint64_t l = lo >= 64 ? ~0ULL : ((1ULL << lo) - 1ULL);
+  return h & ~l;
+}
+
+// Mask over the OTHER spin's orbitals whose bits lie strictly between the
+// spin orbitals 2u+s and 2w+s (u < w) in the interleaved determinant:
+// s = 0 (alpha excitation) -> beta orbitals [u, w); s = 1 -> alpha orbitals [u+1, w+1).
+__host__ __device__ inline uint64_t other_mask(int s, int u, int w) {
+  return s == 0 ? bits_range(u, w) : bits_range(u + 1, w + 1);
+}
+
+// Other-spin sign mask of a record (singles: one range, doubles: two ranges).
+__host__ __device__ inline uint64_t rec_other_mask(int s, uint32_t c, bool dbl) {
+  int x0 = code_x0(c), y0 = code_y0(c);
+  uint64_t m = other_mask(s, x0 < y0 ? x0 : y0, x0 < y0 ? y0 : x0);
+  if (dbl) {
+    int x1 = code_x1(c), y1 = code_y1(c);
+    m ^= other_mask(s, x1 < y1 ? x1 : y1, x1 < y1 ? y1 : x1);
+  }
+  return m;
+}
+
+__host__ __device__ inline uint64_t spread32(uint64_t v) {
+  v &= 0xFFFFFFFFULL;
+  v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
+  v = (v | (v << 8)) & 0x00FF00FF00FF00FFULL;
+  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+  v = (v | (v << 2)) & 0x3333333333333333ULL;
+  v = (v | (v << 1)) & 0x5555555555555555ULL;
+  return v;
+}
+
+// packed 8-fold ERI index (chemistry/basic/integrals.h:94-102), spatial orbitals
+__host__ __device__ inline int64_t eri_index(int I, int J, int K, int L) {
+  int mij = I > J ? I : J, nij = I > J ? J : I;
+  int mkl = K > L ? K : L, nkl = K > L ? L : K;
+  int64_t ij = (int64_t)mij * (mij + 1) / 2 + nij;
+  int64_t kl = (int64_t)mkl * (mkl + 1) / 2 + nkl;
+  int64_t a = ij > kl ? ij : kl, b = ij > kl ? kl : ij;
+  return a * (a + 1) / 2 + b;
+}
+
+// twoInt::Value over spin orbitals: zero unless spin(i)==spin(j) and spin(k)==spin(l)
+__host__ __device__ inline double eri_value(const double* I2, int i, int j, int k, int l) {
+  if (!((i % 2 == j % 2) && (k % 2 == l % 2))) return 0.0;
+  return I2[eri_index(i / 2, j / 2, k / 2, l / 2)];
+}
+
+// One spin's excitation tables on the device (CSR by bra string).
+struct SpinTables {
+  uint32_t* s_off = nullptr;  // n+1
+  uint32_t* d_off = nullptr;  // n+1
+  PairRec* s_rec = nullptr;
+  PairRec* d_rec = nullptr;
+  uint64_t n_s = 0, n_d = 0;
+  uint32_t max_s = 0, max_d = 0;  // max records per bra
+};
+
+}  // namespace sbdx
+
+// The opaque context behind the C ABI.
+struct sbd_ctx {
+  int norb = 0;
+  int device = 0;
+  uint64_t na = 0, nb = 0, N = 0;
+  double I0 = 0.0;
+  std::vector<double> h1;    // L*L one-body (spatial)
+  std::vector<double> eri;   // packed
+  std::vector<uint64_t> a_host, b_host;
+  // device copies
+  uint64_t* d_a = nullptr;
+  uint64_t* d_b = nullptr;
+  double* d_I1 = nullptr;   // (2L)x(2L) spin-orbital one-body, index i*2L+j
+  double* d_eri = nullptr;  // packed ERI
+  double* d_J = nullptr;    // L x L  (pp|qq)
+  double* d_K = nullptr;    // L x L  (pq|qp)
+  double* d_G = nullptr;    // 2 x L x L x 2L: single-excitation integral sums per spin
+  sbdx::SpinTables ta, tb;
+  bool tables_built = false;
+  // scratch for reductions
+  double* d_partials = nullptr;
+  double* d_results = nullptr;
+  double* h_results = nullptr;  // pinned
+  size_t partial_cap = 0;
+  int lds_max = 65536;
+  int cu_count = 256;
+  std::string last_error;
+};
+
+#define SBD_HIP_TRY(ctx, expr)                                                        \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      if (ctx) (ctx)->last_error = std::string(#expr) + ": " + hipGetErrorString(_e); \
+      return SBD_ERR_DEVICE;                                                          \
+    }                                                                                 \
+  } while (0)
+
+// kernel launchers (defined in the .hip files)
+namespace sbdx {
+int build_tables(sbd_ctx* ctx, hipStream_t st);
+int build_gtable(sbd_ctx* ctx, hipStream_t st);
+int launch_hdiag(sbd_ctx* ctx, double* hii, hipStream_t st);
+int launch_hdiag_f32(sbd_ctx* ctx, float* hii, hipStream_t st);
+template <typename VT>
+int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st);
+
+// Davidson vector kernels; all reductions are deterministic (fixed grid,
+// per-thread Kahan sums, fixed-order final sum) and accumulate in FP64.
+constexpr int kMaxVecs = 40;
+template <typename VT>
+int vec_multidot(sbd_ctx* ctx, const VT* const* X, int m, const VT* y, uint64_t n, double* out,
+                 hipStream_t st);
+template <typename VT>
+int vec_ritz_precond(sbd_ctx* ctx, const VT* const* C, const VT* const* HC, int k, const double* u,
+                     double E0, const VT* dii, VT* t, uint64_t n, double* out, hipStream_t st);
+template <typename VT>
+int vec_gs_update(sbd_ctx* ctx, const VT* const* C, int k, const double* coef, VT* t, double scale,
+                  uint64_t n, double* out_norm2, hipStream_t st);
+template <typename VT>
+int vec_lincomb(sbd_ctx* ctx, const VT* const* C, int k, const double* u, VT* out, uint64_t n,
+                double* out_norm2, hipStream_t st);
+template <typename VT>
+int vec_scale(sbd_ctx* ctx, VT* x, double s, uint64_t n, hipStream_t st);
+template <typename VS, typename VD>
+int vec_convert(sbd_ctx* ctx, const VS* x, VD* y, uint64_t n, hipStream_t st);
+}  // namespace sbdx

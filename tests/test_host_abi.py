@@ -1,0 +1,125 @@
+"""CPU tests of the C ABI library: it loads, exports every function that
+include/sbd_tpb.h declares, and its host-only entry points (string packing,
+FCIDUMP parsing, argument validation) behave like the reference's host code.
+No compute call is made here (there is no GPU in this container)."""
+import gzip
+import os
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+
+@pytest.fixture(scope="module")
+def L():
+    import sbd_amd
+    return sbd_amd.lib()
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "sbd_tpb.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sbd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_symbol(L):
+    import sbd_amd
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), f"{n} not exported"
+    assert sorted(sbd_amd.EXPORTS) == names
+
+
+def test_pack_strings_kat(L):
+    """from_string_checked("10000001", bit_length 4, 8) -> words {1, 8}
+    (test_initial_determinant.cc:16-24) packs back to 0b10000001."""
+    import sbd_amd
+    words = np.array([[1, 8]], dtype=np.uint64)
+    assert sbd_amd.pack_strings(words, 4, 8).tolist() == [0b10000001]
+    # default bit_length 20, norb 36 (two words per string)
+    rng = np.random.default_rng(1)
+    vals = [int(x) & ((1 << 36) - 1) for x in rng.integers(0, 2 ** 62, 50, dtype=np.int64)]
+    w = np.array([[v & ((1 << 20) - 1), v >> 20] for v in vals], dtype=np.uint64)
+    assert sbd_amd.pack_strings(w, 20, 36).tolist() == vals
+
+
+def test_from_string_matches_reference_rule():
+    import sbd_amd
+    assert sbd_amd.from_string("000000000000000000011111", 24) == 0b11111
+    assert sbd_amd.from_string("100", 3) == 0b100
+
+
+def test_fcidump_parse_matches_oracle(L, tmp_path):
+    import sbd_amd
+    from oracle import oracle as O
+    src = os.path.join(GOLDEN, "n2", "fcidump.txt.gz")
+    plain = tmp_path / "FCIDUMP"
+    with gzip.open(src, "rb") as fi, open(plain, "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    fd = sbd_amd.LoadFCIDump(str(plain))
+    hdr, vals, idx = O.load_fcidump(src)
+    assert fd.norb == int(hdr["NORB"]) == 16 or fd.norb == int(hdr["NORB"])
+    assert fd.nelec == int(hdr["NELEC"])
+    assert np.array_equal(fd.values, vals)
+    assert np.array_equal(fd.idx, idx)
+
+
+def test_missing_file_is_io_error(L):
+    import sbd_amd
+    with pytest.raises(sbd_amd.SbdError, match="i/o"):
+        sbd_amd.LoadFCIDump("/nonexistent/FCIDUMP")
+
+
+def test_argument_validation_before_device(L):
+    """Bad inputs are rejected with SBD_ERR_ARG before any device call."""
+    import ctypes as C
+    import sbd_amd
+    st = C.c_int(-1)
+    a = np.array([0b0011, 0b0101], dtype=np.uint64)
+    h = np.zeros(16)
+    eri = np.zeros(55)
+    P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    # norb out of range
+    r = L.sbd_ctx_create(65, 0.0, P(h, C.c_double), P(eri, C.c_double), P(a, C.c_uint64), 2,
+                         P(a, C.c_uint64), 2, C.byref(st))
+    assert not r and st.value == 1
+    # unsorted strings (reference lists are sorted, unique)
+    bad = np.array([0b0101, 0b0011], dtype=np.uint64)
+    r = L.sbd_ctx_create(4, 0.0, P(h, C.c_double), P(eri, C.c_double), P(bad, C.c_uint64), 2,
+                         P(a, C.c_uint64), 2, C.byref(st))
+    assert not r and st.value == 1
+    # string with bits beyond norb
+    big = np.array([0b0011, 0b10001], dtype=np.uint64)
+    r = L.sbd_ctx_create(4, 0.0, P(h, C.c_double), P(eri, C.c_double), P(big, C.c_uint64), 2,
+                         P(a, C.c_uint64), 2, C.byref(st))
+    assert not r and st.value == 1
+    # FCIDUMP entry with an orbital index > norb
+    vals = np.array([1.0])
+    idx = np.array([5, 1, 0, 0], dtype=np.int32)
+    r = L.sbd_ctx_create_fcidump(4, 1, P(vals, C.c_double), P(idx, C.c_int), P(a, C.c_uint64), 2,
+                                 P(a, C.c_uint64), 2, C.byref(st))
+    assert not r and st.value == 1
+    # status strings
+    assert L.sbd_status_string(0) == b"ok"
+    assert L.sbd_status_string(4) == b"out of device memory"
+
+
+def test_options_default_match_reference(L):
+    """sbd::tpb::SBD defaults (chemistry/tpb/sbdiag.h:30-45)."""
+    import sbd_amd
+    o = sbd_amd.SBD().options()
+    assert (o.method, o.max_it, o.max_nb, o.init, o.seed) == (0, 1, 10, 0, 1729)
+    assert o.eps == 1e-4 and o.max_time == 86400.0
+
+
+def test_no_cpu_fallback_without_library(monkeypatch, tmp_path):
+    """The product path fails loudly when the HIP library is missing."""
+    import sbd_amd
+    monkeypatch.setattr(sbd_amd, "_LIB", None)
+    monkeypatch.setattr(sbd_amd, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(sbd_amd.SbdError, match="not built"):
+        sbd_amd.lib()

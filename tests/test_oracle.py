@@ -1,0 +1,172 @@
+"""CPU tests that pin the oracle (the CPU restatement of the reference TPB path).
+
+Pins, each chosen so a plausible slip (dropped term, wrong sign or index,
+transposed operand) fails one of them:
+* brute force: the full CI matrix built from second quantisation
+  (oracle/brute.py, no shared code) equals the oracle's H element by element;
+* the reference's own known answers: DetFromAlphaBeta interleave KATs
+  (tests/functionality/test_determinants.cc:123-146), from_string_checked
+  ("10000001", bit_length 4 -> {1, 8}; test_initial_determinant.cc:16-24),
+  InnerProduct/Normalize KATs (test_inner_product.cc:16-81);
+* reference-held energies: data/h2o/README.md:6 and data/n2/README.md:6;
+* closed forms: FCI excitation counts n(L-n) and C(n,2)C(L-n,2) per string;
+  the subspace eigensolver against numpy.linalg.eigh.
+"""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle
+    oracle.lib()
+    return oracle
+
+
+@pytest.mark.parametrize("L,na,nb,seed", [(5, 2, 2, 7), (6, 3, 2, 11), (6, 2, 3, 5), (7, 1, 2, 3)])
+def test_oracle_matches_second_quantisation(O, L, na, nb, seed):
+    from oracle import brute
+    from sbd_amd import synth
+    _, h, eri = synth.make_integrals(L, seed, 2 * L)
+    a, b = synth.all_strings(L, na), synth.all_strings(L, nb)
+    S = O.System.from_workload(dict(norb=L, I0=1.25, h=h, eri=eri, a=a, b=b))
+    hii = S.hdiag()
+    H = np.zeros((S.N, S.N))
+    for j in range(S.N):
+        e = np.zeros(S.N)
+        e[j] = 1.0
+        H[:, j] = S.sigma(hii, e)
+    Hb = brute.dense_hamiltonian(L, 1.25, h, synth.unpack_eri(eri, L), a, b)
+    assert np.abs(H - Hb).max() < 1e-12 * max(1.0, np.abs(Hb).max())
+    assert np.array_equal(H, H.T)
+    # a sign error anywhere would change the spectrum
+    assert np.abs(np.linalg.eigvalsh(H) - np.linalg.eigvalsh(Hb)).max() < 1e-11
+
+
+def test_det_from_alpha_beta_kats(O):
+    """test_determinants.cc:123-146 (bit_length 64, L = 4 and the Fe4S4 regression)."""
+    import ctypes as C
+
+    def det(a, b, L):
+        A = np.array([a], dtype=np.uint64)
+        B = np.array([b], dtype=np.uint64)
+        D = np.zeros(2, dtype=np.uint64)
+        O.lib().orc_det_from_ab(O._p(A, C.c_uint64), O._p(B, C.c_uint64), 1, L,
+                                O._p(D, C.c_uint64))
+        return int(D[0]) | (int(D[1]) << 64)
+
+    assert det(0, 0, 4) == 0
+    assert det(0b1111, 0b1111, 4) == 0b11111111
+    assert det(0b1010, 0, 4) == 0b01000100
+    assert det(0, 0b1010, 4) == 0b10001000
+    assert bin(det((1 << 24) - 1, (1 << 20) - 1, 36)).count("1") == 44
+
+
+def test_from_string_kat(O):
+    """from_string_checked("10000001", 4, 8) -> words {1, 8} (test_initial_determinant.cc:16-24)."""
+    v = O.from_string("10000001", 8)
+    assert v == 0b10000001
+    words = [(v >> (4 * k)) & 0xF for k in range(2)]
+    assert words == [1, 8]
+
+
+def test_inner_product_kats(O):
+    """test_inner_product.cc:16-81."""
+    x = np.array([1.0, 0, 0, 0])
+    y = np.array([0.0, 1, 0, 0])
+    assert O.inner(x, y) == 0.0
+    assert abs(O.inner(np.array([1.0, 2, 3, 4]), np.array([2.0, 4, 6, 8])) - 60.0) < 1e-9 * 60
+    v = np.full(4, 1.0 / math.sqrt(4.0))
+    assert abs(O.inner(v, v) - 1.0) < 1e-9
+    i = np.arange(100000, dtype=np.float64)
+    X, Y = np.sin(i * 0.001), np.cos(i * 0.001)
+    exp = sum((X * Y).tolist())
+    assert abs(O.inner(X, Y) - exp) <= 1e-10 * abs(exp)
+    j = np.arange(81796, dtype=np.float64)
+    V = np.sin(j * 0.0001) * np.exp(-j * 1e-6)
+    Vn, _ = O.normalize(V)
+    assert abs(O.inner(Vn, Vn) - 1.0) < 1e-9
+
+
+def test_fci_table_counts_closed_form(O):
+    from sbd_amd import synth
+    for L, n in [(8, 3), (10, 4)]:
+        a = synth.all_strings(L, n)
+        _, h, eri = synth.make_integrals(L, 1, 2)
+        S = O.System.from_workload(dict(norb=L, I0=0.0, h=h, eri=eri, a=a, b=a))
+        c = S.counts()
+        assert c[0] == len(a) * n * (L - n)
+        assert c[1] == len(a) * math.comb(n, 2) * math.comb(L - n, 2)
+        off, ket, cr = S.table(1)
+        # kets ascending per bra, CrAn = (bra-only x0 < x1, ket-only y0 < y1), even (alpha)
+        for i in range(0, len(a), 17):
+            ks = ket[off[i]:off[i + 1]]
+            assert np.all(np.diff(ks) > 0)
+            for k, q in zip(ks, cr[off[i]:off[i + 1]]):
+                xa, xb = int(a[i]), int(a[k])
+                assert [2 * p for p in range(L) if (xa & ~xb) >> p & 1] == list(q[:2])
+                assert [2 * p for p in range(L) if (xb & ~xa) >> p & 1] == list(q[2:])
+
+
+def test_syev_matches_numpy(O):
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 5, 10, 20):
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        w, U = O.syev(A)
+        w_ref = np.linalg.eigvalsh(A)
+        assert np.abs(w - w_ref).max() < 1e-12 * max(1, np.abs(w_ref).max())
+        assert np.abs(A @ U - U * w).max() < 1e-11 * max(1, np.abs(w_ref).max())
+        assert np.abs(U.T @ U - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("mol,alpha,E_known", [
+    ("h2o", "h2o-1em3-alpha.txt.gz", -76.23594663),
+    ("n2", "1em3-alpha.txt.gz", -109.04162110),
+])
+def test_oracle_reference_energies(O, mol, alpha, E_known):
+    hdr, vals, idx = O.load_fcidump(os.path.join(GOLDEN, mol, "fcidump.txt.gz"))
+    L, _, I0, I1, I2, Dm, Xm = O.setup_integrals(hdr, vals, idx)
+    a = O.load_alpha_dets(os.path.join(GOLDEN, mol, alpha), L)
+    S = O.System(L, a, a, I0, I1, I2, Dm, Xm)
+    hii = S.hdiag()
+    W0 = np.zeros(S.N)
+    W0[0] = 1.0
+    W, hist = S.davidson(hii, W0, 20, 10, 1e-8)
+    assert hist[-1][1] < 1e-8
+    E = S.energy(hii, W)
+    assert abs(E - E_known) < 6e-9, (E, E_known)
+
+
+def test_synthetic_generator_is_pinned():
+    """The seeded generator is part of the benchmark definition: its outputs are
+    pinned by digests committed in tests/golden/synth_digests.json."""
+    from sbd_amd import synth
+    ref = json.load(open(os.path.join(GOLDEN, "synth_digests.json")))
+    for name, d in ref.items():
+        w = synth.make_workload(name, d["scale"])
+        got = {
+            "h": hashlib.sha256(np.ascontiguousarray(w["h"]).tobytes()).hexdigest(),
+            "eri": hashlib.sha256(np.ascontiguousarray(w["eri"]).tobytes()).hexdigest(),
+            "a": hashlib.sha256(w["a"].tobytes()).hexdigest(),
+            "b": hashlib.sha256(w["b"].tobytes()).hexdigest(),
+        }
+        for k in got:
+            assert got[k] == d[k], (name, k)
+        assert len(w["a"]) == d["na"] and len(w["b"]) == d["nb"]
+
+
+def test_golden_energies_consistent(O):
+    """The committed oracle energies are below every diagonal element
+    (variational) and converged (residual < 1e-8)."""
+    g = json.load(open(os.path.join(GOLDEN, "energies.json")))
+    for key, d in g.items():
+        assert d["residual"] < 1e-8
+        assert d["energy"] < d["hii_min"]
