@@ -536,7 +536,10 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
   void* ptrs[] = {ctx->d_a,       ctx->d_b,       ctx->d_I1,      ctx->d_eri,     ctx->d_J,
                   ctx->d_K,       ctx->d_G,       ctx->ta.s_off,  ctx->ta.d_off,  ctx->ta.s_rec,
                   ctx->ta.d_rec,  ctx->tb.s_off,  ctx->tb.d_off,  ctx->tb.s_rec,  ctx->tb.d_rec,
-                  ctx->d_partials, ctx->d_results};
+                  ctx->d_partials, ctx->d_results, ctx->ta.with_s, ctx->tb.with_s,
+                  ctx->ta.ell_ket, ctx->ta.ell_val, ctx->ta.ell_mask, ctx->ta.ell_off,
+                  ctx->tb.ell_ket, ctx->tb.ell_val, ctx->tb.ell_mask, ctx->tb.ell_off,
+                  ctx->ta.s_bra,   ctx->tb.s_bra,   ctx->d_Ea,       ctx->d_Eb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_results) (void)hipHostFree(ctx->h_results);
@@ -556,6 +559,7 @@ int sbd_make_helpers(sbd_ctx* ctx) {
   if (!ctx) return SBD_ERR_ARG;
   if (ctx->tables_built) return SBD_OK;
   int rc = build_tables(ctx, nullptr);
+  if (!rc) rc = build_single_cache(ctx, nullptr);
   if (rc) return rc;
   ctx->tables_built = true;
   return SBD_OK;
@@ -588,18 +592,23 @@ int sbd_table_get(const sbd_ctx* ctx, int which, int64_t* off, int64_t* ket, int
                        hipMemcpyDeviceToHost) != hipSuccess)
     return SBD_ERR_DEVICE;
   for (uint64_t i = 0; i <= n; i++) off[i] = o[i];
-  for (uint64_t q = 0; q < cnt; q++) {
-    if (ket) ket[q] = r[q].ket;
-    if (cran) {
-      const uint32_t c = r[q].code;
+  const std::vector<uint64_t>& S = which < 2 ? ctx->a_host : ctx->b_host;
+  for (uint64_t i = 0; i < n; i++) {
+    for (uint64_t q = o[i]; q < o[i + 1]; q++) {
+      if (ket) ket[q] = r[q].ket;
+      if (!cran) continue;
       if (dbl) {
-        cran[4 * q + 0] = 2 * code_x0(c) + spin;
-        cran[4 * q + 1] = 2 * code_x1(c) + spin;
-        cran[4 * q + 2] = 2 * code_y0(c) + spin;
-        cran[4 * q + 3] = 2 * code_y1(c) + spin;
+        // OrbitalDifference (determinants.h:406-449): bra-only, then ket-only, ascending
+        uint64_t x = S[i] & ~S[r[q].ket], y = S[r[q].ket] & ~S[i];
+        const int x0 = __builtin_ctzll(x), x1 = __builtin_ctzll(x & (x - 1));
+        const int y0 = __builtin_ctzll(y), y1 = __builtin_ctzll(y & (y - 1));
+        cran[4 * q + 0] = 2 * x0 + spin;
+        cran[4 * q + 1] = 2 * x1 + spin;
+        cran[4 * q + 2] = 2 * y0 + spin;
+        cran[4 * q + 3] = 2 * y1 + spin;
       } else {
-        cran[2 * q + 0] = 2 * code_x0(c) + spin;
-        cran[2 * q + 1] = 2 * code_y0(c) + spin;
+        cran[2 * q + 0] = 2 * code_x0(r[q].code) + spin;
+        cran[2 * q + 1] = 2 * code_y0(r[q].code) + spin;
       }
     }
   }

@@ -20,11 +20,14 @@ namespace sbdx {
 // Excitation record: one (bra, ket) pair of strings of one spin that differ by
 // a single (2 bits) or double (4 bits) excitation.  16 bytes.
 //   ket  : ket string index
-//   code : bra-only orbitals x0 (bits 0-5), x1 (6-11); ket-only orbitals
-//          y0 (12-17), y1 (18-23), each ascending; bit 24: same-spin parity of
-//          a single excitation (number of bra bits of this spin strictly between
-//          x0 and y0, mod 2)
-//   val  : single: one-body integral h[y0][x0] (= I1.Value(a,i));
+//   code : single: bra-only orbital x (bits 0-5), ket-only orbital y (12-17),
+//          bit 24 = same-spin parity (bra bits of this spin strictly between
+//          x and y, mod 2);
+//          double: the two orbital ranges that the sign depends on,
+//          lo1 = min(x0,y0) (0-5), hi1 = max(x0,y0) (6-11), lo2 = min(x1,y1)
+//          (12-17), hi2 = max(x1,y1) (18-23), with x0 < x1 bra-only and
+//          y0 < y1 ket-only orbitals
+//   val  : single: one-body integral h[y][x] (= I1.Value(a,i));
 //          double: same-spin sign x extra sign x ((AI|BJ) - (AJ|BI))
 struct alignas(16) PairRec {
   uint32_t ket;
@@ -33,10 +36,9 @@ struct alignas(16) PairRec {
 };
 
 __host__ __device__ inline int code_x0(uint32_t c) { return c & 63; }
-__host__ __device__ inline int code_x1(uint32_t c) { return (c >> 6) & 63; }
 __host__ __device__ inline int code_y0(uint32_t c) { return (c >> 12) & 63; }
-__host__ __device__ inline int code_y1(uint32_t c) { return (c >> 18) & 63; }
 __host__ __device__ inline int code_par(uint32_t c) { return (c >> 24) & 1; }
+__host__ __device__ inline int code_f(uint32_t c, int k) { return (c >> (6 * k)) & 63; }
 
 // bits [lo, hi) of a 64-bit word, 0 <= lo <= hi <= 64
 __host__ __device__ inline uint64_t bits_range(int lo, int hi) {
@@ -45,22 +47,37 @@ __host__ __device__ inline uint64_t bits_range(int lo, int hi) {
   return h & ~l;
 }
 
-// Mask over the OTHER spin's orbitals whose bits lie strictly between the
-// spin orbitals 2u+s and 2w+s (u < w) in the interleaved determinant:
-// s = 0 (alpha excitation) -> beta orbitals [u, w); s = 1 -> alpha orbitals [u+1, w+1).
-__host__ __device__ inline uint64_t other_mask(int s, int u, int w) {
-  return s == 0 ? bits_range(u, w) : bits_range(u + 1, w + 1);
+// The sign of an excitation between spin orbitals 2u+s and 2w+s (u < w)
+// picks up the parity of the OTHER spin's bits strictly between them in the
+// interleaved determinant: s = 0 -> beta orbitals [u, w); s = 1 -> alpha
+// orbitals [u+1, w+1).
+//
+// Alpha excitations (other = beta): mask over beta orbitals.
+__host__ __device__ inline uint64_t beta_mask_single(uint32_t c) {
+  const int x = code_x0(c), y = code_y0(c);
+  return bits_range(x < y ? x : y, x < y ? y : x);
 }
-
-// Other-spin sign mask of a record (singles: one range, doubles: two ranges).
-__host__ __device__ inline uint64_t rec_other_mask(int s, uint32_t c, bool dbl) {
-  int x0 = code_x0(c), y0 = code_y0(c);
-  uint64_t m = other_mask(s, x0 < y0 ? x0 : y0, x0 < y0 ? y0 : x0);
-  if (dbl) {
-    int x1 = code_x1(c), y1 = code_y1(c);
-    m ^= other_mask(s, x1 < y1 ? x1 : y1, x1 < y1 ? y1 : x1);
-  }
-  return m;
+__host__ __device__ inline uint64_t beta_mask_double(uint32_t c) {
+  return bits_range(code_f(c, 0), code_f(c, 1)) ^ bits_range(code_f(c, 2), code_f(c, 3));
+}
+// Beta excitations (other = alpha): with PA = inclusive prefix parity of the
+// alpha string (bit j = parity of alpha bits 0..j), the parity of alpha
+// orbitals [u+1, w+1) is bit u ^ bit w of PA.
+__host__ __device__ inline uint64_t prefix_parity(uint64_t x) {
+  x ^= x << 1;
+  x ^= x << 2;
+  x ^= x << 4;
+  x ^= x << 8;
+  x ^= x << 16;
+  x ^= x << 32;
+  return x;
+}
+__host__ __device__ inline uint32_t alpha_par_single(uint64_t PA, uint32_t c) {
+  return (uint32_t)((PA >> code_x0(c)) ^ (PA >> code_y0(c))) & 1u;
+}
+__host__ __device__ inline uint32_t alpha_par_double(uint64_t PA, uint32_t c) {
+  return (uint32_t)((PA >> code_f(c, 0)) ^ (PA >> code_f(c, 1)) ^ (PA >> code_f(c, 2)) ^
+                    (PA >> code_f(c, 3))) & 1u;
 }
 
 __host__ __device__ inline uint64_t spread32(uint64_t v) {
@@ -97,7 +114,23 @@ struct SpinTables {
   PairRec* d_rec = nullptr;
   uint64_t n_s = 0, n_d = 0;
   uint32_t max_s = 0, max_d = 0;  // max records per bra
+  uint32_t* with_s = nullptr;     // ascending list of strings that have single excitations
+  uint32_t n_with_s = 0;
+  // Doubles in sliced-ELL order for per-column access by a wavefront: slice s
+  // holds columns [64s, 64s+64); record u of column 64s+l is at slot
+  // ell_off[s] + 64u + l for u < count(column) (slots past a column's count
+  // repeat its last record).  Stored as three arrays: ket index, value, and
+  // the other-spin sign mask (the element is -val when popcount(other & mask)
+  // is odd).
+  uint32_t* ell_ket = nullptr;
+  double* ell_val = nullptr;
+  uint64_t* ell_mask = nullptr;
+  uint32_t* ell_off = nullptr;  // n_slices + 1
+  uint32_t n_slices = 0;
+  uint32_t* s_bra = nullptr;    // bra string of each single record
 };
+
+constexpr int kEllSlice = 64;
 
 }  // namespace sbdx
 
@@ -120,6 +153,14 @@ struct sbd_ctx {
   double* d_G = nullptr;    // 2 x L x L x 2L: single-excitation integral sums per spin
   sbdx::SpinTables ta, tb;
   bool tables_built = false;
+  // Single-excitation matrix elements, precomputed once per context (same
+  // operations as the on-the-fly evaluation, so bit-identical):
+  //   Ea[q * nb + ib] : alpha single record q (bra row s_bra[q]) at beta column ib
+  //   Eb[ia * Sb + q] : beta single record q (bra column s_bra[q]) at alpha row ia
+  // Left null (on-the-fly evaluation) when they would exceed the cache budget.
+  double* d_Ea = nullptr;
+  double* d_Eb = nullptr;
+  uint64_t single_cache_budget = 8ULL << 30;
   // scratch for reductions
   double* d_partials = nullptr;
   double* d_results = nullptr;
@@ -143,6 +184,7 @@ struct sbd_ctx {
 namespace sbdx {
 int build_tables(sbd_ctx* ctx, hipStream_t st);
 int build_gtable(sbd_ctx* ctx, hipStream_t st);
+int build_single_cache(sbd_ctx* ctx, hipStream_t st);
 int launch_hdiag(sbd_ctx* ctx, double* hii, hipStream_t st);
 int launch_hdiag_f32(sbd_ctx* ctx, float* hii, hipStream_t st);
 template <typename VT>

@@ -13,6 +13,8 @@
 // excitations of (ia, ib) land in that row.  Alpha-pair records are
 // workgroup-uniform (scalar loads); beta-pair records are per column.  There
 // are no atomics: each output element has one writer.
+#include <type_traits>
+
 #include "sbd_internal.hpp"
 
 namespace sbdx {
@@ -111,23 +113,42 @@ struct SigmaArgs {
   const uint32_t* bs_off;
   const PairRec* bs_rec;
   const uint32_t* bd_off;
-  const PairRec* bd_rec;
+  const uint32_t* bd_ell_ket;  // beta doubles, sliced-ELL order
+  const double* bd_ell_val;
+  const uint64_t* bd_ell_mask;
+  const uint32_t* bd_ell_off;
+  const double* Ea;  // alpha single elements [q][ib] (or null)
+  const double* Eb;  // beta single elements [ia][q] (or null)
+  uint32_t Sb;       // number of beta single records
   const double* G;   // [2][L][L][2L]
   const double* I2;  // packed ERI
 };
 
 // OneExcite energy (before the sign): val + sum over occupied spin orbitals j
-// of the bra, ascending, of G[j].
+// of the bra, ascending, of G[j].  The gathers are issued kGBatch at a time and
+// then added in ascending order, so the result is the sequential sum.
+constexpr int kGBatch = 8;
 __device__ inline double one_excite_sum(double val, const double* __restrict__ g, uint64_t D0,
                                         uint64_t D1) {
   double e = val;
-  while (D0) {
-    e += g[__ffsll((unsigned long long)D0) - 1];
-    D0 &= D0 - 1;
-  }
-  while (D1) {
-    e += g[__ffsll((unsigned long long)D1) - 1 + 64];
-    D1 &= D1 - 1;
+#pragma unroll
+  for (int w = 0; w < 2; w++) {
+    uint64_t x = w ? D1 : D0;
+    while (x) {
+      double gv[kGBatch];
+      int cnt = 0;
+#pragma unroll
+      for (int k = 0; k < kGBatch; k++) {
+        if (x) {
+          gv[k] = g[__ffsll((unsigned long long)x) - 1 + 64 * w];
+          x &= x - 1;
+          cnt = k + 1;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kGBatch; k++)
+        if (k < cnt) e += gv[k];
+    }
   }
   return e;
 }
@@ -153,94 +174,186 @@ __device__ inline double two_excite_full(uint64_t D0, uint64_t D1, int i, int j,
   return sgn * (eri_value(I2, A, I, B, J) - eri_value(I2, A, J, B, I));
 }
 
-constexpr int kSigBlock = 512;
-constexpr int kBatch = 8;
+// Single-excitation elements with their full sign (OneExcite, determinants.h:
+// 476-498, on the determinant (alpha string A, beta string B)).
+__device__ inline double alpha_single_elem(const PairRec& r, uint64_t A, uint64_t B, int L,
+                                           const double* __restrict__ G) {
+  const uint64_t D0 = spread32(A) | (spread32(B) << 1);
+  const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
+  const double* g = G + ((int64_t)code_x0(r.code) * L + code_y0(r.code)) * (2 * L);
+  double e = one_excite_sum(r.val, g, D0, D1);
+  if (code_par(r.code) ^ (__popcll(B & beta_mask_single(r.code)) & 1)) e = -e;
+  return e;
+}
+__device__ inline double beta_single_elem(const PairRec& r, uint64_t A, uint64_t B, int L,
+                                          const double* __restrict__ G) {
+  const uint64_t D0 = spread32(A) | (spread32(B) << 1);
+  const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
+  const double* g =
+      G + (int64_t)L * L * 2 * L + ((int64_t)code_x0(r.code) * L + code_y0(r.code)) * (2 * L);
+  double e = one_excite_sum(r.val, g, D0, D1);
+  if (code_par(r.code) ^ alpha_par_single(prefix_parity(A), r.code)) e = -e;
+  return e;
+}
 
+// Ea[q * nb + ib] for every alpha single record q and beta string ib.
+__global__ void k_single_alpha(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
+                               uint64_t nb, uint64_t total, int L,
+                               const PairRec* __restrict__ rec, const uint32_t* __restrict__ bra,
+                               const double* __restrict__ G, double* __restrict__ Ea) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = t / nb, ib = t - q * nb;
+    Ea[t] = alpha_single_elem(rec[q], a[bra[q]], b[ib], L, G);
+  }
+}
+
+// Eb[ia * Sb + q] for every alpha string ia and beta single record q.
+__global__ void k_single_beta(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
+                              uint64_t Sb, uint64_t total, int L, const PairRec* __restrict__ rec,
+                              const uint32_t* __restrict__ bra, const double* __restrict__ G,
+                              double* __restrict__ Eb) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t ia = t / Sb, q = t - ia * Sb;
+    Eb[t] = beta_single_elem(rec[q], a[ia], b[bra[q]], L, G);
+  }
+}
+
+constexpr int kSigBlock = 1024;  // 16 waves: one workgroup per CU when the row fills LDS
+constexpr int kBatch = 8;        // alpha pairs whose ket loads are issued together
+constexpr int kBBatch = 4;       // beta double slots fetched together
+
+template <typename VT>
+__device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ Trow, uint32_t nb) {
+  uint32_t j = threadIdx.x;
+  for (; j + 3 * kSigBlock < nb; j += 4 * kSigBlock) {
+    const VT v0 = Trow[j], v1 = Trow[j + kSigBlock], v2 = Trow[j + 2 * kSigBlock],
+             v3 = Trow[j + 3 * kSigBlock];
+    trow[j] = v0;
+    trow[j + kSigBlock] = v1;
+    trow[j + 2 * kSigBlock] = v2;
+    trow[j + 3 * kSigBlock] = v3;
+  }
+  for (; j < nb; j += kSigBlock) trow[j] = Trow[j];
+}
+
+// One workgroup per bra alpha row (grid-stride over rows), one beta column
+// per thread at a time, so a wavefront covers one 64-column slice of the
+// beta tables.  Each element is accumulated in the reference order.  Every
+// pointer is a restrict-qualified kernel argument: W aliases no input, so the
+// workgroup-uniform alpha records are fetched with scalar loads.
 template <typename VT, typename AT, bool kRowLds>
-__global__ __launch_bounds__(kSigBlock) void k_sigma_rows(const SigmaArgs<VT> p) {
+__global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
+    const uint64_t* __restrict__ a_, const uint64_t* __restrict__ b_, uint64_t na_, uint64_t nb_,
+    int L_, const VT* __restrict__ hii_, const VT* __restrict__ T_, VT* __restrict__ W_,
+    int accumulate_, const uint32_t* __restrict__ as_off_, const PairRec* __restrict__ as_rec_,
+    const uint32_t* __restrict__ ad_off_, const PairRec* __restrict__ ad_rec_,
+    const uint32_t* __restrict__ bs_off_, const PairRec* __restrict__ bs_rec_,
+    const uint32_t* __restrict__ bd_off_, const uint32_t* __restrict__ bd_ell_ket_,
+    const double* __restrict__ bd_ell_val_, const uint64_t* __restrict__ bd_ell_mask_,
+    const uint32_t* __restrict__ bd_ell_off_, const double* __restrict__ Ea_,
+    const double* __restrict__ Eb_, uint32_t Sb_, const double* __restrict__ G_,
+    const double* __restrict__ I2_) {
+  const SigmaArgs<VT> p{a_,      b_,          na_,          nb_,          L_,     hii_,
+                        T_,      W_,          accumulate_,  as_off_,      as_rec_, ad_off_,
+                        ad_rec_, bs_off_,     bs_rec_,      bd_off_,      bd_ell_ket_,
+                        bd_ell_val_, bd_ell_mask_, bd_ell_off_, Ea_,      Eb_,     Sb_,
+                        G_,      I2_};
   extern __shared__ unsigned char lds_raw[];
   VT* trow = reinterpret_cast<VT*>(lds_raw);
-  const uint64_t nb = p.nb;
+  const uint32_t nb = (uint32_t)p.nb;
   const int L = p.L;
-  const int64_t GL = (int64_t)L * L * 2 * L;  // one spin's G block
   for (uint64_t ia = blockIdx.x; ia < p.na; ia += gridDim.x) {
     const VT* __restrict__ Trow = p.T + ia * nb;
+    const VT* __restrict__ Hrow = p.hii + ia * nb;
+    VT* __restrict__ Wrow = p.W + ia * nb;
     if (kRowLds) {
       __syncthreads();
-      for (uint64_t ib = threadIdx.x; ib < nb; ib += blockDim.x) trow[ib] = Trow[ib];
+      stage_row<VT>(trow, Trow, nb);
       __syncthreads();
     }
+    const VT* __restrict__ Tsrc = kRowLds ? trow : Trow;
     const uint64_t A = p.a[ia];
-    const uint64_t A0 = spread32(A), A1 = spread32(A >> 32);
     const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
     const uint32_t ad0 = p.ad_off[ia], ad1 = p.ad_off[ia + 1];
-    for (uint64_t ib = threadIdx.x; ib < nb; ib += blockDim.x) {
+    const double* __restrict__ Ebrow = p.Eb ? p.Eb + ia * (uint64_t)p.Sb : nullptr;
+    for (uint32_t ib = threadIdx.x; ib < nb; ib += kSigBlock) {
+      const VT hv = Hrow[ib];
       const uint64_t B = p.b[ib];
-      const uint64_t D0 = A0 | (spread32(B) << 1);
-      const uint64_t D1 = A1 | (spread32(B >> 32) << 1);
-      const uint64_t idx = ia * nb + ib;
-      AT acc = p.accumulate ? (AT)p.W[idx] : (AT)0;
-      const VT tself = kRowLds ? trow[ib] : Trow[ib];
-      acc += (AT)p.hii[idx] * (AT)tself;
+      const uint32_t bs0 = p.bs_off[ib], bs1 = p.bs_off[ib + 1];
+      const uint32_t bd0 = p.bd_off[ib], bd1 = p.bd_off[ib + 1];
+      const VT ts = Tsrc[ib];
+      AT acc = p.accumulate ? (AT)Wrow[ib] : (AT)0;
+      acc += (AT)hv * (AT)ts;
 
-      // alpha singles (task type 2, OneExcite)
+      // alpha singles (task type 2, OneExcite): workgroup-uniform records
       for (uint32_t q = as0; q < as1; q++) {
         const PairRec r = p.as_rec[q];
-        const int x = code_x0(r.code), y = code_y0(r.code);
-        const double* g = p.G + ((int64_t)x * L + y) * (2 * L);
-        double e = one_excite_sum(r.val, g, D0, D1);
-        const int par = code_par(r.code) ^ (__popcll(B & rec_other_mask(0, r.code, false)) & 1);
-        if (par) e = -e;
-        acc += (AT)e * (AT)p.T[(uint64_t)r.ket * nb + ib];
+        const VT tk = p.T[(uint64_t)r.ket * nb + ib];
+        const double e = p.Ea ? p.Ea[(uint64_t)q * nb + ib] : alpha_single_elem(r, A, B, L, p.G);
+        acc += (AT)e * (AT)tk;
       }
-      // alpha doubles (task type 2, TwoExcite): loads batched ahead of the
-      // in-order accumulation
+      // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
       for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
         VT tv[kBatch];
-        double ev[kBatch];
+        double vv[kBatch];
+        uint64_t mm[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
           if (q0 + u < ad1) {
             const PairRec r = p.ad_rec[q0 + u];
             tv[u] = p.T[(uint64_t)r.ket * nb + ib];
-            ev[u] = (__popcll(B & rec_other_mask(0, r.code, true)) & 1) ? -r.val : r.val;
+            vv[u] = r.val;
+            mm[u] = beta_mask_double(r.code);
           }
         }
-#pragma unroll
-        for (int u = 0; u < kBatch; u++)
-          if (q0 + u < ad1) acc += (AT)ev[u] * (AT)tv[u];
-      }
-      // beta singles (task type 1, OneExcite)
-      const uint32_t bs0 = p.bs_off[ib], bs1 = p.bs_off[ib + 1];
-      for (uint32_t q = bs0; q < bs1; q++) {
-        const PairRec r = p.bs_rec[q];
-        const int x = code_x0(r.code), y = code_y0(r.code);
-        const double* g = p.G + GL + ((int64_t)x * L + y) * (2 * L);
-        double e = one_excite_sum(r.val, g, D0, D1);
-        const int par = code_par(r.code) ^ (__popcll(A & rec_other_mask(1, r.code, false)) & 1);
-        if (par) e = -e;
-        const VT t = kRowLds ? trow[r.ket] : Trow[r.ket];
-        acc += (AT)e * (AT)t;
-      }
-      // beta doubles (task type 1, TwoExcite)
-      const uint32_t bd0 = p.bd_off[ib], bd1 = p.bd_off[ib + 1];
-      for (uint32_t q0 = bd0; q0 < bd1; q0 += kBatch) {
-        VT tv[kBatch];
-        double ev[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
-          if (q0 + u < bd1) {
-            const PairRec r = p.bd_rec[q0 + u];
-            tv[u] = kRowLds ? trow[r.ket] : Trow[r.ket];
-            ev[u] = (__popcll(A & rec_other_mask(1, r.code, true)) & 1) ? -r.val : r.val;
+          if (q0 + u < ad1) {
+            const double e = (__popcll(B & mm[u]) & 1) ? -vv[u] : vv[u];
+            acc += (AT)e * (AT)tv[u];
           }
         }
+      }
+      // beta singles (task type 1, OneExcite): rare, per column
+      for (uint32_t q = bs0; q < bs1; q++) {
+        const PairRec r = p.bs_rec[q];
+        const double e = Ebrow ? Ebrow[q] : beta_single_elem(r, A, B, L, p.G);
+        acc += (AT)e * (AT)Tsrc[r.ket];
+      }
+      // beta doubles (task type 1, TwoExcite), sliced-ELL: slot k of this
+      // column is at base + 64k; slots past the count re-read a valid record
+      // and their sum is discarded with a select (bit-identical to skipping)
+      {
+        const uint32_t cnt = bd1 - bd0;
+        const size_t base = p.bd_ell_off[ib / kEllSlice] + ib % kEllSlice;
+        for (uint32_t k0 = 0; k0 < cnt; k0 += kBBatch) {
+          uint32_t kk[kBBatch];
+          double vv[kBBatch];
+          uint64_t mm[kBBatch];
+          VT tt[kBBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++)
-          if (q0 + u < bd1) acc += (AT)ev[u] * (AT)tv[u];
+          for (int u = 0; u < kBBatch; u++) {
+            const uint32_t k = k0 + u < cnt ? k0 + u : cnt - 1;
+            kk[u] = p.bd_ell_ket[base + (size_t)k * kEllSlice];
+            vv[u] = p.bd_ell_val[base + (size_t)k * kEllSlice];
+            mm[u] = p.bd_ell_mask[base + (size_t)k * kEllSlice];
+          }
+#pragma unroll
+          for (int u = 0; u < kBBatch; u++) tt[u] = Tsrc[kk[u]];
+#pragma unroll
+          for (int u = 0; u < kBBatch; u++) {
+            const double e = (__popcll(A & mm[u]) & 1) ? -vv[u] : vv[u];
+            const AT nv = acc + (AT)e * (AT)tt[u];
+            acc = (k0 + u < cnt) ? nv : acc;
+          }
+        }
       }
       // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by))
-      if (bs1 > bs0) {
+      if (bs1 > bs0 && as1 > as0) {
+        const uint64_t D0 = spread32(A) | (spread32(B) << 1);
+        const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
         for (uint32_t q = as0; q < as1; q++) {
           const PairRec ra = p.as_rec[q];
           const int ai = 2 * code_x0(ra.code), aa = 2 * code_y0(ra.code);
@@ -253,9 +366,33 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(const SigmaArgs<VT> p)
           }
         }
       }
-      p.W[idx] = (VT)acc;
+      __builtin_nontemporal_store((VT)acc, Wrow + ib);
     }
   }
+}
+
+int build_single_cache(sbd_ctx* ctx, hipStream_t st) {
+  const uint64_t Sa = ctx->ta.n_s, Sb = ctx->tb.n_s;
+  const uint64_t bytes = 8 * (Sa * ctx->nb + ctx->na * Sb);
+  if (bytes == 0 || bytes > ctx->single_cache_budget) return SBD_OK;  // evaluate on the fly
+  if (Sa) {
+    SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_Ea, 8 * Sa * ctx->nb));
+    const uint64_t total = Sa * ctx->nb;
+    const unsigned grid = (unsigned)std::min<uint64_t>((total + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_single_alpha, dim3(grid), dim3(256), 0, st, ctx->d_a, ctx->d_b, ctx->nb,
+                       total, ctx->norb, ctx->ta.s_rec, ctx->ta.s_bra, ctx->d_G, ctx->d_Ea);
+    SBD_HIP_TRY(ctx, hipGetLastError());
+  }
+  if (Sb) {
+    SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_Eb, 8 * ctx->na * Sb));
+    const uint64_t total = ctx->na * Sb;
+    const unsigned grid = (unsigned)std::min<uint64_t>((total + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_single_beta, dim3(grid), dim3(256), 0, st, ctx->d_a, ctx->d_b, Sb,
+                       total, ctx->norb, ctx->tb.s_rec, ctx->tb.s_bra, ctx->d_G, ctx->d_Eb);
+    SBD_HIP_TRY(ctx, hipGetLastError());
+  }
+  SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
+  return SBD_OK;
 }
 
 template <typename VT>
@@ -264,34 +401,34 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
     ctx->last_error = "sigma called before sbd_make_helpers";
     return SBD_ERR_STATE;
   }
-  SigmaArgs<VT> p;
-  p.a = ctx->d_a;
-  p.b = ctx->d_b;
-  p.na = ctx->na;
-  p.nb = ctx->nb;
-  p.L = ctx->norb;
-  p.hii = hii;
-  p.T = T;
-  p.W = W;
-  p.accumulate = accumulate;
-  p.as_off = ctx->ta.s_off;
-  p.as_rec = ctx->ta.s_rec;
-  p.ad_off = ctx->ta.d_off;
-  p.ad_rec = ctx->ta.d_rec;
-  p.bs_off = ctx->tb.s_off;
-  p.bs_rec = ctx->tb.s_rec;
-  p.bd_off = ctx->tb.d_off;
-  p.bd_rec = ctx->tb.d_rec;
-  p.G = ctx->d_G;
-  p.I2 = ctx->d_eri;
   if (ctx->na == 0 || ctx->nb == 0) return SBD_OK;
-  const size_t row_bytes = sizeof(VT) * ctx->nb;
-  unsigned grid = (unsigned)(ctx->na < 0x7FFFFFFFULL ? ctx->na : 0x7FFFFFFFULL);
-  if (row_bytes <= (size_t)ctx->lds_max) {
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, true>), dim3(grid), dim3(kSigBlock), row_bytes, st, p);
-  } else {
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, false>), dim3(grid), dim3(kSigBlock), 0, st, p);
+  // the output may not overlap the input vector or hii (the kernel's pointers
+  // are restrict-qualified, and rows of T are read after W rows are written)
+  const uintptr_t w0 = (uintptr_t)W, w1 = w0 + sizeof(VT) * ctx->N;
+  for (uintptr_t x : {(uintptr_t)T, (uintptr_t)hii}) {
+    if (x < w1 && w0 < x + sizeof(VT) * ctx->N) {
+      ctx->last_error = "sigma: Wb overlaps Wk or hii";
+      return SBD_ERR_ARG;
+    }
   }
+  const size_t row_bytes = sizeof(VT) * ctx->nb;
+  const bool lds = row_bytes <= (size_t)ctx->lds_max;
+  const unsigned grid = (unsigned)(ctx->na < 0x7FFFFFFFULL ? ctx->na : 0x7FFFFFFFULL);
+  const size_t shm = lds ? row_bytes : 0;
+  const SpinTables& A = ctx->ta;
+  const SpinTables& B = ctx->tb;
+#define SBD_SIGMA_ARGS                                                                         \
+  ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,    \
+      A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_val, B.ell_mask,           \
+      B.ell_off, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,         \
+      (const double*)ctx->d_G, (const double*)ctx->d_eri
+  if (lds)
+    hipLaunchKernelGGL((k_sigma_rows<VT, VT, true>), dim3(grid), dim3(kSigBlock), shm, st,
+                       SBD_SIGMA_ARGS);
+  else
+    hipLaunchKernelGGL((k_sigma_rows<VT, VT, false>), dim3(grid), dim3(kSigBlock), 0, st,
+                       SBD_SIGMA_ARGS);
+#undef SBD_SIGMA_ARGS
   SBD_HIP_TRY(ctx, hipGetLastError());
   return SBD_OK;
 }
@@ -305,10 +442,10 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
   int lds = 65536;
   SBD_HIP_TRY(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                          ctx->device));
-  (void)hipFuncSetAttribute((const void*)k_sigma_rows<double, double, true>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  (void)hipFuncSetAttribute((const void*)k_sigma_rows<float, float, true>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  const void* fns[] = {(const void*)k_sigma_rows<double, double, true>,
+                       (const void*)k_sigma_rows<float, float, true>};
+  for (const void* f : fns)
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipGetLastError();
   ctx->lds_max = lds;
   return SBD_OK;

@@ -12,6 +12,8 @@
 // parity.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "sbd_internal.hpp"
 
 namespace sbdx {
@@ -63,7 +65,9 @@ __device__ inline PairRec make_rec(uint64_t bra, uint64_t ket, uint32_t ket_idx,
     yb &= yb - 1;
     const int x1 = __ffsll((unsigned long long)xb) - 1;
     const int y1 = __ffsll((unsigned long long)yb) - 1;
-    r.code = (uint32_t)x0 | ((uint32_t)x1 << 6) | ((uint32_t)y0 << 12) | ((uint32_t)y1 << 18);
+    const uint32_t lo1 = x0 < y0 ? x0 : y0, hi1 = x0 < y0 ? y0 : x0;
+    const uint32_t lo2 = x1 < y1 ? x1 : y1, hi2 = x1 < y1 ? y1 : x1;
+    r.code = lo1 | (hi1 << 6) | (lo2 << 12) | (hi2 << 18);
     // TwoExcite (determinants.h:500-518) with i=2x0+s, j=2x1+s, a=2y0+s, b=2y1+s:
     // I=i, J=j, A=a, B=b since x0<x1, y0<y1.
     const int I = 2 * x0 + s, J = 2 * x1 + s, A = 2 * y0 + s, B = 2 * y1 + s;
@@ -100,6 +104,40 @@ __global__ __launch_bounds__(kTabBlock) void k_tab_fill(const uint64_t* __restri
       if (d == 2) s_rec[ps++] = make_rec(me, o, t0 + j, s_spin, 2 * L, I1, I2, false);
       else if (d == 4) d_rec[pd++] = make_rec(me, o, t0 + j, s_spin, 2 * L, I1, I2, true);
     }
+  }
+}
+
+// Sliced-ELL copy of a doubles CSR table (one thread per bra string), with
+// the other-spin sign mask of each record expanded from its code.
+__global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off,
+                          const PairRec* __restrict__ rec, const uint32_t* __restrict__ ell_off,
+                          uint32_t* __restrict__ ell_ket, double* __restrict__ ell_val,
+                          uint64_t* __restrict__ ell_mask) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = i / kEllSlice, l = i % kEllSlice;
+  const uint32_t o0 = off[i], cnt = off[i + 1] - o0;
+  const uint32_t w = (ell_off[s + 1] - ell_off[s]) / kEllSlice;
+  for (uint32_t u = 0; u < w; u++) {
+    const uint32_t k = u < cnt ? u : (cnt ? cnt - 1 : 0);
+    uint32_t ket = 0;
+    double val = 0.0;
+    uint64_t mask = 0;
+    if (cnt) {
+      const PairRec r = rec[o0 + k];
+      ket = r.ket;
+      val = r.val;
+      const uint32_t c = r.code;
+      // other spin's orbitals between the excitation's spin orbitals:
+      // alpha excitation -> beta [lo, hi); beta excitation -> alpha [lo+1, hi+1)
+      const int d = spin == 0 ? 0 : 1;
+      mask = bits_range(code_f(c, 0) + d, code_f(c, 1) + d) ^
+             bits_range(code_f(c, 2) + d, code_f(c, 3) + d);
+    }
+    const size_t slot = ell_off[s] + (size_t)u * kEllSlice + l;
+    ell_ket[slot] = ket;
+    ell_val[slot] = val;
+    ell_mask[slot] = mask;
   }
 }
 
@@ -173,6 +211,41 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
   T.n_d = tot_d;
   T.max_s = mx_s;
   T.max_d = mx_d;
+  // strings with at least one single excitation
+  std::vector<uint32_t> ws;
+  for (uint32_t i = 0; i < n; i++)
+    if (hs[i]) ws.push_back(i);
+  T.n_with_s = (uint32_t)ws.size();
+  SBD_HIP_TRY(ctx, hipMalloc(&T.with_s, sizeof(uint32_t) * (ws.size() + 1)));
+  if (!ws.empty())
+    SBD_HIP_TRY(ctx, hipMemcpy(T.with_s, ws.data(), sizeof(uint32_t) * ws.size(),
+                               hipMemcpyHostToDevice));
+  // sliced-ELL copy of the doubles
+  T.n_slices = (n + kEllSlice - 1) / kEllSlice;
+  std::vector<uint32_t> eoff(T.n_slices + 1, 0);
+  for (uint32_t s = 0; s < T.n_slices; s++) {
+    uint32_t w = 0;
+    for (uint32_t i = s * kEllSlice; i < std::min(n, (s + 1) * kEllSlice); i++) w = std::max(w, hd[i]);
+    eoff[s + 1] = eoff[s] + w * kEllSlice;
+  }
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_off, sizeof(uint32_t) * (T.n_slices + 1)));
+  SBD_HIP_TRY(ctx, hipMemcpy(T.ell_off, eoff.data(), sizeof(uint32_t) * (T.n_slices + 1),
+                             hipMemcpyHostToDevice));
+  const size_t nslot = (size_t)eoff[T.n_slices] + 1;
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_ket, sizeof(uint32_t) * nslot));
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_val, sizeof(double) * nslot));
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_mask, sizeof(uint64_t) * nslot));
+  hipLaunchKernelGGL(k_tab_ell, dim3((n + 255) / 256), dim3(256), 0, st, n, spin, T.d_off,
+                     T.d_rec, T.ell_off, T.ell_ket, T.ell_val, T.ell_mask);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  // bra string of every single record
+  std::vector<uint32_t> sb(tot_s + 1, 0);
+  for (uint32_t i = 0, q = 0; i < n; i++)
+    for (uint32_t k = 0; k < hs[i]; k++) sb[q++] = i;
+  SBD_HIP_TRY(ctx, hipMalloc(&T.s_bra, sizeof(uint32_t) * sb.size()));
+  SBD_HIP_TRY(ctx, hipMemcpy(T.s_bra, sb.data(), sizeof(uint32_t) * sb.size(),
+                             hipMemcpyHostToDevice));
+  SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
   return SBD_OK;
 }
 

@@ -59,6 +59,13 @@ class SplitMix64:
         return z[:n]
 
 
+def random_init_vector(N: int, seed: int) -> np.ndarray:
+    """The initial vector of sbd_diag with init = 1: splitmix64(seed) uniforms
+    mapped to (-1, 1), normalised."""
+    w = 2.0 * SplitMix64(seed).uniform(N) - 1.0
+    return w / np.sqrt(np.dot(w, w))
+
+
 def eri_index(i, j, k, l):
     ij = max(i, j) * (max(i, j) + 1) // 2 + min(i, j)
     kl = max(k, l) * (max(k, l) + 1) // 2 + min(k, l)
@@ -156,6 +163,10 @@ WORKLOADS = {
     "sci36": dict(norb=36, na=8, nb=8, ca=20000, cb=20000, rank=72, seed=2002),
     "sci58": dict(norb=58, na=10, nb=10, ca=60000, cb=60000, rank=116, seed=3003),
     "sci48open": dict(norb=48, na=12, nb=10, ca=30000, cb=25000, rank=96, seed=5005),
+    # small, densely connected selected bases for Davidson parity (in the sparse
+    # workloads above most determinants are nearly isolated at small scale)
+    "sci16": dict(norb=16, na=4, nb=4, ca=400, cb=400, rank=32, seed=6006),
+    "sci16open": dict(norb=16, na=5, nb=3, ca=400, cb=300, rank=32, seed=7007),
 }
 
 
