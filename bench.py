@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-davidson", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--dav-restarts", type=int, default=60,
+                    help="Davidson max_it (restarts of max_nb=10 steps each)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -180,14 +182,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": B,
-                     "kernel": "k_sigma_rows<double,double,true,2>"},
+                     "kernel": "k_sigma_rows<double, double, true>"},
         "setup_s": {"tables": round(t_tables, 3), "hdiag": round(t_hdiag, 3)},
     }
 
     if not args.no_davidson:
         res = {}
         for label, prec in (("fp64", 0), ("mixed", 1)):
-            opt = sbd_amd.SBD(max_it=10, max_nb=10, eps=1e-7, precision=prec, fp32_eps=1e-4)
+            opt = sbd_amd.SBD(max_it=args.dav_restarts, max_nb=10, eps=1e-7, precision=prec,
+                              fp32_eps=1e-4)
             t0 = time.time()
             E, _, rep, hist = m.diag(opt, history=True)
             res[label] = {"energy": E, "steps": rep["steps"], "steps_fp32": rep["steps_fp32"],
