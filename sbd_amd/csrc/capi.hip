@@ -537,8 +537,9 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
                   ctx->d_K,       ctx->d_G,       ctx->ta.s_off,  ctx->ta.d_off,  ctx->ta.s_rec,
                   ctx->ta.d_rec,  ctx->tb.s_off,  ctx->tb.d_off,  ctx->tb.s_rec,  ctx->tb.d_rec,
                   ctx->d_partials, ctx->d_results, ctx->ta.with_s, ctx->tb.with_s,
-                  ctx->ta.ell_ket, ctx->ta.ell_val, ctx->ta.ell_mask, ctx->ta.ell_off,
-                  ctx->tb.ell_ket, ctx->tb.ell_val, ctx->tb.ell_mask, ctx->tb.ell_off,
+                  ctx->ta.ell_ket, ctx->ta.ell_vm, ctx->ta.ell_off,
+                  ctx->tb.ell_ket, ctx->tb.ell_vm, ctx->tb.ell_off,
+                  ctx->ta.ell_col, ctx->tb.ell_col,
                   ctx->ta.s_bra,   ctx->tb.s_bra,   ctx->d_Ea,       ctx->d_Eb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -106,6 +106,12 @@ __host__ __device__ inline double eri_value(const double* I2, int i, int j, int 
   return I2[eri_index(i / 2, j / 2, k / 2, l / 2)];
 }
 
+// Value and other-spin sign mask of one doubles slot, fetched as one 16-byte load.
+struct alignas(16) ValMask {
+  double val;
+  uint64_t mask;
+};
+
 // One spin's excitation tables on the device (CSR by bra string).
 struct SpinTables {
   uint32_t* s_off = nullptr;  // n+1
@@ -116,21 +122,24 @@ struct SpinTables {
   uint32_t max_s = 0, max_d = 0;  // max records per bra
   uint32_t* with_s = nullptr;     // ascending list of strings that have single excitations
   uint32_t n_with_s = 0;
-  // Doubles in sliced-ELL order for per-column access by a wavefront: slice s
-  // holds columns [64s, 64s+64); record u of column 64s+l is at slot
-  // ell_off[s] + 64u + l for u < count(column) (slots past a column's count
-  // repeat its last record).  Stored as three arrays: ket index, value, and
+  // Doubles in sliced-ELL order for per-column access by a wavefront.  The
+  // strings are taken in blocks of kColBlock; within a block, positions list
+  // the block's strings in descending order of their doubles count (stable),
+  // ell_col[position] = string.  Slice s holds positions [64s, 64s+64);
+  // record u of position 64s+l is at slot ell_off[s] + 64u + l for u below
+  // that string's count (slots past the count repeat its last record).  Stored as two arrays: ket index, and value with
   // the other-spin sign mask (the element is -val when popcount(other & mask)
   // is odd).
   uint32_t* ell_ket = nullptr;
-  double* ell_val = nullptr;
-  uint64_t* ell_mask = nullptr;
+  ValMask* ell_vm = nullptr;
   uint32_t* ell_off = nullptr;  // n_slices + 1
+  uint32_t* ell_col = nullptr;  // n: string at each block-sorted position
   uint32_t n_slices = 0;
   uint32_t* s_bra = nullptr;    // bra string of each single record
 };
 
 constexpr int kEllSlice = 64;
+constexpr int kColBlock = 1024;  // columns per sigma workgroup step (= its thread count)
 
 }  // namespace sbdx
 

@@ -114,9 +114,9 @@ struct SigmaArgs {
   const PairRec* bs_rec;
   const uint32_t* bd_off;
   const uint32_t* bd_ell_ket;  // beta doubles, sliced-ELL order
-  const double* bd_ell_val;
-  const uint64_t* bd_ell_mask;
+  const ValMask* bd_ell_vm;
   const uint32_t* bd_ell_off;
+  const uint32_t* bd_ell_col;  // column at each block-sorted position
   const double* Ea;  // alpha single elements [q][ib] (or null)
   const double* Eb;  // beta single elements [ia][q] (or null)
   uint32_t Sb;       // number of beta single records
@@ -220,7 +220,7 @@ __global__ void k_single_beta(const uint64_t* __restrict__ a, const uint64_t* __
   }
 }
 
-constexpr int kSigBlock = 1024;  // 16 waves: one workgroup per CU when the row fills LDS
+constexpr int kSigBlock = kColBlock;  // 16 waves: one workgroup per CU when the row fills LDS
 constexpr int kBatch = 8;        // alpha pairs whose ket loads are issued together
 constexpr int kBBatch = 4;       // beta double slots fetched together
 
@@ -238,12 +238,21 @@ __device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ T
   for (; j < nb; j += kSigBlock) trow[j] = Trow[j];
 }
 
-// One workgroup per bra alpha row (grid-stride over rows), one beta column
-// per thread at a time, so a wavefront covers one 64-column slice of the
-// beta tables.  Each element is accumulated in the reference order.  Every
-// pointer is a restrict-qualified kernel argument: W aliases no input, so the
+// One workgroup per bra alpha row (grid-stride over rows); the beta columns
+// are taken kSigBlock at a time.  Each element is accumulated in the
+// reference order, in two phases per block of columns:
+//   1. thread t owns column blk + t: diagonal, alpha singles, alpha doubles
+//      and beta singles; the partial sum is stored to W;
+//   2. after a barrier, thread t owns the column at block-sorted position
+//      blk + t (the block's columns in descending order of their beta-double
+//      count), reloads its partial sum and adds the beta doubles and the
+//      alpha-single x beta-single terms.
+// Phase 1 keeps every access to T, hii and W coalesced; phase 2 gives each
+// wavefront 64 columns of nearly equal beta-double count, so its ELL slice
+// carries little padding and its lanes finish together.  Every pointer is a
+// restrict-qualified kernel argument: W aliases no input, so the
 // workgroup-uniform alpha records are fetched with scalar loads.
-template <typename VT, typename AT, bool kRowLds>
+template <typename VT, typename AT, bool kRowLds, bool kCache>
 __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint64_t* __restrict__ a_, const uint64_t* __restrict__ b_, uint64_t na_, uint64_t nb_,
     int L_, const VT* __restrict__ hii_, const VT* __restrict__ T_, VT* __restrict__ W_,
@@ -251,25 +260,27 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint32_t* __restrict__ ad_off_, const PairRec* __restrict__ ad_rec_,
     const uint32_t* __restrict__ bs_off_, const PairRec* __restrict__ bs_rec_,
     const uint32_t* __restrict__ bd_off_, const uint32_t* __restrict__ bd_ell_ket_,
-    const double* __restrict__ bd_ell_val_, const uint64_t* __restrict__ bd_ell_mask_,
-    const uint32_t* __restrict__ bd_ell_off_, const double* __restrict__ Ea_,
+    const ValMask* __restrict__ bd_ell_vm_, const uint32_t* __restrict__ bd_ell_off_,
+    const uint32_t* __restrict__ bd_ell_col_, const double* __restrict__ Ea_,
     const double* __restrict__ Eb_, uint32_t Sb_, const double* __restrict__ G_,
     const double* __restrict__ I2_) {
-  const SigmaArgs<VT> p{a_,      b_,          na_,          nb_,          L_,     hii_,
-                        T_,      W_,          accumulate_,  as_off_,      as_rec_, ad_off_,
-                        ad_rec_, bs_off_,     bs_rec_,      bd_off_,      bd_ell_ket_,
-                        bd_ell_val_, bd_ell_mask_, bd_ell_off_, Ea_,      Eb_,     Sb_,
-                        G_,      I2_};
+  static_assert(std::is_same<VT, AT>::value, "partial sums round-trip through W");
+  const SigmaArgs<VT> p{a_,      b_,          na_,         nb_,         L_,          hii_,
+                        T_,      W_,          accumulate_, as_off_,     as_rec_,     ad_off_,
+                        ad_rec_, bs_off_,     bs_rec_,     bd_off_,     bd_ell_ket_, bd_ell_vm_,
+                        bd_ell_off_, bd_ell_col_, Ea_,     Eb_,         Sb_,         G_,
+                        I2_};
   extern __shared__ unsigned char lds_raw[];
   VT* trow = reinterpret_cast<VT*>(lds_raw);
   const uint32_t nb = (uint32_t)p.nb;
   const int L = p.L;
+  const uint32_t nblk = (nb + kSigBlock - 1) / kSigBlock;
   for (uint64_t ia = blockIdx.x; ia < p.na; ia += gridDim.x) {
     const VT* __restrict__ Trow = p.T + ia * nb;
     const VT* __restrict__ Hrow = p.hii + ia * nb;
     VT* __restrict__ Wrow = p.W + ia * nb;
+    __syncthreads();
     if (kRowLds) {
-      __syncthreads();
       stage_row<VT>(trow, Trow, nb);
       __syncthreads();
     }
@@ -278,56 +289,66 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
     const uint32_t ad0 = p.ad_off[ia], ad1 = p.ad_off[ia + 1];
     const double* __restrict__ Ebrow = p.Eb ? p.Eb + ia * (uint64_t)p.Sb : nullptr;
-    for (uint32_t ib = threadIdx.x; ib < nb; ib += kSigBlock) {
-      const VT hv = Hrow[ib];
-      const uint64_t B = p.b[ib];
-      const uint32_t bs0 = p.bs_off[ib], bs1 = p.bs_off[ib + 1];
-      const uint32_t bd0 = p.bd_off[ib], bd1 = p.bd_off[ib + 1];
-      const VT ts = Tsrc[ib];
-      AT acc = p.accumulate ? (AT)Wrow[ib] : (AT)0;
-      acc += (AT)hv * (AT)ts;
-
-      // alpha singles (task type 2, OneExcite): workgroup-uniform records
-      for (uint32_t q = as0; q < as1; q++) {
-        const PairRec r = p.as_rec[q];
-        const VT tk = p.T[(uint64_t)r.ket * nb + ib];
-        const double e = p.Ea ? p.Ea[(uint64_t)q * nb + ib] : alpha_single_elem(r, A, B, L, p.G);
-        acc += (AT)e * (AT)tk;
-      }
-      // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
-      for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
-        VT tv[kBatch];
-        double vv[kBatch];
-        uint64_t mm[kBatch];
+    for (uint32_t g = 0; g < nblk; g++) {
+      const uint32_t ib = g * kSigBlock + threadIdx.x;
+      // ---- phase 1: column ib
+      if (ib < nb) {
+        const VT hv = Hrow[ib];
+        const uint64_t B = p.b[ib];
+        const uint32_t bs0 = p.bs_off[ib], bs1 = p.bs_off[ib + 1];
+        const VT ts = Tsrc[ib];
+        AT acc = p.accumulate ? (AT)Wrow[ib] : (AT)0;
+        acc += (AT)hv * (AT)ts;
+        // alpha singles (task type 2, OneExcite): workgroup-uniform records
+        for (uint32_t q = as0; q < as1; q++) {
+          const PairRec r = p.as_rec[q];
+          const VT tk = p.T[(uint64_t)r.ket * nb + ib];
+          const double e =
+              kCache ? p.Ea[(uint64_t)q * nb + ib] : alpha_single_elem(r, A, B, L, p.G);
+          acc += (AT)e * (AT)tk;
+        }
+        // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
+        for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
+          VT tv[kBatch];
+          double vv[kBatch];
+          uint64_t mm[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-          if (q0 + u < ad1) {
-            const PairRec r = p.ad_rec[q0 + u];
-            tv[u] = p.T[(uint64_t)r.ket * nb + ib];
-            vv[u] = r.val;
-            mm[u] = beta_mask_double(r.code);
+          for (int u = 0; u < kBatch; u++) {
+            if (q0 + u < ad1) {
+              const PairRec r = p.ad_rec[q0 + u];
+              tv[u] = p.T[(uint64_t)r.ket * nb + ib];
+              vv[u] = r.val;
+              mm[u] = beta_mask_double(r.code);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kBatch; u++) {
+            if (q0 + u < ad1) {
+              const double e = (__popcll(B & mm[u]) & 1) ? -vv[u] : vv[u];
+              acc += (AT)e * (AT)tv[u];
+            }
           }
         }
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-          if (q0 + u < ad1) {
-            const double e = (__popcll(B & mm[u]) & 1) ? -vv[u] : vv[u];
-            acc += (AT)e * (AT)tv[u];
-          }
+        // beta singles (task type 1, OneExcite): rare, per column
+        for (uint32_t q = bs0; q < bs1; q++) {
+          const PairRec r = p.bs_rec[q];
+          const double e = kCache ? Ebrow[q] : beta_single_elem(r, A, B, L, p.G);
+          acc += (AT)e * (AT)Tsrc[r.ket];
         }
+        Wrow[ib] = (VT)acc;
       }
-      // beta singles (task type 1, OneExcite): rare, per column
-      for (uint32_t q = bs0; q < bs1; q++) {
-        const PairRec r = p.bs_rec[q];
-        const double e = Ebrow ? Ebrow[q] : beta_single_elem(r, A, B, L, p.G);
-        acc += (AT)e * (AT)Tsrc[r.ket];
-      }
-      // beta doubles (task type 1, TwoExcite), sliced-ELL: slot k of this
-      // column is at base + 64k; slots past the count re-read a valid record
-      // and their sum is discarded with a select (bit-identical to skipping)
-      {
-        const uint32_t cnt = bd1 - bd0;
-        const size_t base = p.bd_ell_off[ib / kEllSlice] + ib % kEllSlice;
+      __syncthreads();
+      // ---- phase 2: the column at block-sorted position ib
+      if (ib < nb) {
+        const uint32_t c = p.bd_ell_col[ib];
+        const uint32_t bd0 = p.bd_off[c], cnt = p.bd_off[c + 1] - bd0;
+        AT acc = (AT)Wrow[c];
+        // beta doubles (task type 1, TwoExcite), sliced-ELL: slot k of this
+        // position is at base + 64k; slots past the count re-read a valid
+        // record and their sum is discarded with a select (bit-identical to
+        // skipping it)
+        const uint32_t sl = __builtin_amdgcn_readfirstlane(ib / kEllSlice);
+        const size_t base = p.bd_ell_off[sl] + ib % kEllSlice;
         for (uint32_t k0 = 0; k0 < cnt; k0 += kBBatch) {
           uint32_t kk[kBBatch];
           double vv[kBBatch];
@@ -337,8 +358,9 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
           for (int u = 0; u < kBBatch; u++) {
             const uint32_t k = k0 + u < cnt ? k0 + u : cnt - 1;
             kk[u] = p.bd_ell_ket[base + (size_t)k * kEllSlice];
-            vv[u] = p.bd_ell_val[base + (size_t)k * kEllSlice];
-            mm[u] = p.bd_ell_mask[base + (size_t)k * kEllSlice];
+            const ValMask vm = p.bd_ell_vm[base + (size_t)k * kEllSlice];
+            vv[u] = vm.val;
+            mm[u] = vm.mask;
           }
 #pragma unroll
           for (int u = 0; u < kBBatch; u++) tt[u] = Tsrc[kk[u]];
@@ -349,24 +371,28 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
             acc = (k0 + u < cnt) ? nv : acc;
           }
         }
-      }
-      // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by))
-      if (bs1 > bs0 && as1 > as0) {
-        const uint64_t D0 = spread32(A) | (spread32(B) << 1);
-        const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
-        for (uint32_t q = as0; q < as1; q++) {
-          const PairRec ra = p.as_rec[q];
-          const int ai = 2 * code_x0(ra.code), aa = 2 * code_y0(ra.code);
-          const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
-          for (uint32_t k = bs0; k < bs1; k++) {
-            const PairRec rb = p.bs_rec[k];
-            const int bi = 2 * code_x0(rb.code) + 1, ba = 2 * code_y0(rb.code) + 1;
-            const double e = two_excite_full(D0, D1, ai, bi, aa, ba, p.I2);
-            acc += (AT)e * (AT)Tk[rb.ket];
+        // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by))
+        if (as1 > as0) {
+          const uint32_t bs0 = p.bs_off[c], bs1 = p.bs_off[c + 1];
+          if (bs1 > bs0) {
+            const uint64_t B = p.b[c];
+            const uint64_t D0 = spread32(A) | (spread32(B) << 1);
+            const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
+            for (uint32_t q = as0; q < as1; q++) {
+              const PairRec ra = p.as_rec[q];
+              const int ai = 2 * code_x0(ra.code), aa = 2 * code_y0(ra.code);
+              const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
+              for (uint32_t k = bs0; k < bs1; k++) {
+                const PairRec rb = p.bs_rec[k];
+                const int bi = 2 * code_x0(rb.code) + 1, ba = 2 * code_y0(rb.code) + 1;
+                const double e = two_excite_full(D0, D1, ai, bi, aa, ba, p.I2);
+                acc += (AT)e * (AT)Tk[rb.ket];
+              }
+            }
           }
         }
+        Wrow[c] = (VT)acc;
       }
-      __builtin_nontemporal_store((VT)acc, Wrow + ib);
     }
   }
 }
@@ -419,14 +445,22 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
   const SpinTables& B = ctx->tb;
 #define SBD_SIGMA_ARGS                                                                         \
   ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,    \
-      A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_val, B.ell_mask,           \
-      B.ell_off, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,         \
+      A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_vm,           \
+      B.ell_off, B.ell_col, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,         \
       (const double*)ctx->d_G, (const double*)ctx->d_eri
-  if (lds)
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, true>), dim3(grid), dim3(kSigBlock), shm, st,
+  // precomputed single-excitation elements, when both caches exist
+  const bool cache = (A.n_s == 0 || ctx->d_Ea) && (B.n_s == 0 || ctx->d_Eb);
+  if (lds && cache)
+    hipLaunchKernelGGL((k_sigma_rows<VT, VT, true, true>), dim3(grid), dim3(kSigBlock), shm, st,
+                       SBD_SIGMA_ARGS);
+  else if (lds)
+    hipLaunchKernelGGL((k_sigma_rows<VT, VT, true, false>), dim3(grid), dim3(kSigBlock), shm, st,
+                       SBD_SIGMA_ARGS);
+  else if (cache)
+    hipLaunchKernelGGL((k_sigma_rows<VT, VT, false, true>), dim3(grid), dim3(kSigBlock), 0, st,
                        SBD_SIGMA_ARGS);
   else
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, false>), dim3(grid), dim3(kSigBlock), 0, st,
+    hipLaunchKernelGGL((k_sigma_rows<VT, VT, false, false>), dim3(grid), dim3(kSigBlock), 0, st,
                        SBD_SIGMA_ARGS);
 #undef SBD_SIGMA_ARGS
   SBD_HIP_TRY(ctx, hipGetLastError());
@@ -442,8 +476,10 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
   int lds = 65536;
   SBD_HIP_TRY(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                          ctx->device));
-  const void* fns[] = {(const void*)k_sigma_rows<double, double, true>,
-                       (const void*)k_sigma_rows<float, float, true>};
+  const void* fns[] = {(const void*)k_sigma_rows<double, double, true, true>,
+                       (const void*)k_sigma_rows<double, double, true, false>,
+                       (const void*)k_sigma_rows<float, float, true, true>,
+                       (const void*)k_sigma_rows<float, float, true, false>};
   for (const void* f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipGetLastError();

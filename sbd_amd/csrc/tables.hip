@@ -107,16 +107,18 @@ __global__ __launch_bounds__(kTabBlock) void k_tab_fill(const uint64_t* __restri
   }
 }
 
-// Sliced-ELL copy of a doubles CSR table (one thread per bra string), with
-// the other-spin sign mask of each record expanded from its code.
+// Sliced-ELL copy of a doubles CSR table (one thread per block-sorted
+// position), with the other-spin sign mask of each record expanded from its
+// code.
 __global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off,
                           const PairRec* __restrict__ rec, const uint32_t* __restrict__ ell_off,
-                          uint32_t* __restrict__ ell_ket, double* __restrict__ ell_val,
-                          uint64_t* __restrict__ ell_mask) {
+                          const uint32_t* __restrict__ ell_col, uint32_t* __restrict__ ell_ket,
+                          ValMask* __restrict__ ell_vm) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = i / kEllSlice, l = i % kEllSlice;
-  const uint32_t o0 = off[i], cnt = off[i + 1] - o0;
+  const uint32_t col = ell_col[i];
+  const uint32_t o0 = off[col], cnt = off[col + 1] - o0;
   const uint32_t w = (ell_off[s + 1] - ell_off[s]) / kEllSlice;
   for (uint32_t u = 0; u < w; u++) {
     const uint32_t k = u < cnt ? u : (cnt ? cnt - 1 : 0);
@@ -136,8 +138,7 @@ __global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off
     }
     const size_t slot = ell_off[s] + (size_t)u * kEllSlice + l;
     ell_ket[slot] = ket;
-    ell_val[slot] = val;
-    ell_mask[slot] = mask;
+    ell_vm[slot] = ValMask{val, mask};
   }
 }
 
@@ -220,12 +221,22 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
   if (!ws.empty())
     SBD_HIP_TRY(ctx, hipMemcpy(T.with_s, ws.data(), sizeof(uint32_t) * ws.size(),
                                hipMemcpyHostToDevice));
-  // sliced-ELL copy of the doubles
+  // sliced-ELL copy of the doubles over block-sorted positions
+  std::vector<uint32_t> col(n);
+  for (uint32_t g0 = 0; g0 < n; g0 += kColBlock) {
+    const uint32_t g1 = std::min<uint64_t>(n, (uint64_t)g0 + kColBlock);
+    for (uint32_t i = g0; i < g1; i++) col[i] = i;
+    std::stable_sort(col.begin() + g0, col.begin() + g1,
+                     [&](uint32_t x, uint32_t y) { return hd[x] > hd[y]; });
+  }
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_col, sizeof(uint32_t) * n));
+  SBD_HIP_TRY(ctx, hipMemcpy(T.ell_col, col.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
   T.n_slices = (n + kEllSlice - 1) / kEllSlice;
   std::vector<uint32_t> eoff(T.n_slices + 1, 0);
   for (uint32_t s = 0; s < T.n_slices; s++) {
     uint32_t w = 0;
-    for (uint32_t i = s * kEllSlice; i < std::min(n, (s + 1) * kEllSlice); i++) w = std::max(w, hd[i]);
+    for (uint32_t i = s * kEllSlice; i < std::min(n, (s + 1) * kEllSlice); i++)
+      w = std::max(w, hd[col[i]]);
     eoff[s + 1] = eoff[s] + w * kEllSlice;
   }
   SBD_HIP_TRY(ctx, hipMalloc(&T.ell_off, sizeof(uint32_t) * (T.n_slices + 1)));
@@ -233,10 +244,9 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
                              hipMemcpyHostToDevice));
   const size_t nslot = (size_t)eoff[T.n_slices] + 1;
   SBD_HIP_TRY(ctx, hipMalloc(&T.ell_ket, sizeof(uint32_t) * nslot));
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_val, sizeof(double) * nslot));
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_mask, sizeof(uint64_t) * nslot));
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_vm, sizeof(ValMask) * nslot));
   hipLaunchKernelGGL(k_tab_ell, dim3((n + 255) / 256), dim3(256), 0, st, n, spin, T.d_off,
-                     T.d_rec, T.ell_off, T.ell_ket, T.ell_val, T.ell_mask);
+                     T.d_rec, T.ell_off, T.ell_col, T.ell_ket, T.ell_vm);
   SBD_HIP_TRY(ctx, hipGetLastError());
   // bra string of every single record
   std::vector<uint32_t> sb(tot_s + 1, 0);
