@@ -243,7 +243,8 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
     HC[k] = HCb[k].p;
   }
   SBD_HIP_TRY(ctx, hipMemcpyAsync(C[0], W, sizeof(VT) * N, hipMemcpyDeviceToDevice, st));
-  std::vector<double> H((size_t)nb * nb, 0.0), U((size_t)nb * nb), E(nb), u(nb), o(kMaxVecs + 3);
+  std::vector<double> H((size_t)nb * nb, 0.0), U((size_t)nb * nb), E(nb), u(nb),
+      o(2 * kMaxVecs + 3);
   std::vector<const VT*> Cc(nb), HCc(nb);
   int steps = 0, rc = 0;
   bool stop = false, converged = false;
@@ -253,6 +254,11 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
   SBD_HIP_TRY(ctx, hipEventCreate(&ev0));
   SBD_HIP_TRY(ctx, hipEventCreate(&ev1));
   int last_k = 1;
+  // <HC_j, C_{ib+1}> for j <= ib, carried from one step to the next: by the
+  // symmetry of H these are the off-diagonal entries of the next projected
+  // column, obtained from the sums of the Ritz pass through the Gram-Schmidt
+  // coefficients, so no overlap pass follows the sigma.
+  std::vector<double> hnext(nb, 0.0), hcur(nb, 0.0);
   for (int it = 0; it < max_it && !stop; it++) {
     for (int ib = 0; ib < nb; ib++) {
       const double ts = now_s();
@@ -261,18 +267,19 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
         HCc[k] = HC[k];
       }
       SBD_HIP_TRY(ctx, hipEventRecord(ev0, st));
-      rc = launch_sigma<VT>(ctx, dii, C[ib], HC[ib], 0, st);
+      double hdiag = 0.0;
+      rc = launch_sigma<VT>(ctx, dii, C[ib], HC[ib], 0, st, &hdiag);
       if (rc) break;
       SBD_HIP_TRY(ctx, hipEventRecord(ev1, st));
-      rc = vec_multidot<VT>(ctx, Cc.data(), ib + 1, HC[ib], N, o.data(), st);
-      if (rc) break;
       float ms = 0.f;
+      SBD_HIP_TRY(ctx, hipEventSynchronize(ev1));
       SBD_HIP_TRY(ctx, hipEventElapsedTime(&ms, ev0, ev1));
       t_sigma += 1e-3 * ms;
-      for (int jb = 0; jb <= ib; jb++) {
-        H[jb + (size_t)nb * ib] = o[jb];
-        H[ib + (size_t)nb * jb] = o[jb];
+      for (int jb = 0; jb < ib; jb++) {
+        H[jb + (size_t)nb * ib] = hnext[jb];
+        H[ib + (size_t)nb * jb] = hnext[jb];
       }
+      H[ib + (size_t)nb * ib] = hdiag;
       for (int jb = 0; jb <= ib; jb++)
         for (int kb = 0; kb <= ib; kb++) U[jb + (size_t)nb * kb] = H[jb + (size_t)nb * kb];
       syev_jacobi(ib + 1, U.data(), nb, E.data());
@@ -303,30 +310,47 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
         break;
       }
       if (ib < nb - 1) {
+        const int k1 = ib + 1;
+        const double* oc = o.data() + 3;       // <C_j, t>
+        const double* oh = o.data() + 3 + k1;  // <HC_j, t>
+        // <HC_j, t - sum_i c_i C_i> = oh_j - sum_i c_i H[i][j]
+        auto project = [&](const double* c, const double* h, double* out) {
+          for (int j = 0; j < k1; j++) {
+            double x = h[j];
+            for (int i = 0; i < k1; i++) x -= c[i] * H[i + (size_t)nb * j];
+            out[j] = x;
+          }
+        };
         const double tt = o[2];
         double proj = tt;
-        for (int k = 0; k <= ib; k++) proj -= o[3 + k] * o[3 + k];
+        for (int k = 0; k < k1; k++) proj -= oc[k] * oc[k];
         double n2 = 0.0;
         if (tt > 0 && proj > 1e-3 * tt) {
-          rc = vec_gs_update<VT>(ctx, Cc.data(), ib + 1, o.data() + 3, t, 1.0 / std::sqrt(proj), N,
-                                 &n2, st);
+          const double sc = 1.0 / std::sqrt(proj);
+          rc = vec_gs_update<VT>(ctx, Cc.data(), k1, oc, t, sc, N, &n2, st);
           if (rc) break;
+          project(oc, oh, hnext.data());
+          for (int j = 0; j < k1; j++) hnext[j] *= sc;
         } else {
-          rc = vec_gs_update<VT>(ctx, Cc.data(), ib + 1, o.data() + 3, t, 1.0, N, &n2, st);
+          // the direction is nearly inside the subspace: orthogonalise twice
+          rc = vec_gs_update<VT>(ctx, Cc.data(), k1, oc, t, 1.0, N, &n2, st);
           if (rc) break;
+          project(oc, oh, hcur.data());
           std::vector<double> o2(kMaxVecs);
-          rc = vec_multidot<VT>(ctx, Cc.data(), ib + 1, t, N, o2.data(), st);
+          rc = vec_multidot<VT>(ctx, Cc.data(), k1, t, N, o2.data(), st);
           if (rc) break;
           double p2 = n2;
-          for (int k = 0; k <= ib; k++) p2 -= o2[k] * o2[k];
+          for (int k = 0; k < k1; k++) p2 -= o2[k] * o2[k];
           if (!(p2 > 0.0)) {
             // the new direction lies in the subspace: nothing left to add
             stop = true;
             break;
           }
-          rc = vec_gs_update<VT>(ctx, Cc.data(), ib + 1, o2.data(), t, 1.0 / std::sqrt(p2), N, &n2,
-                                 st);
+          const double sc = 1.0 / std::sqrt(p2);
+          rc = vec_gs_update<VT>(ctx, Cc.data(), k1, o2.data(), t, sc, N, &n2, st);
           if (rc) break;
+          project(o2.data(), hcur.data(), hnext.data());
+          for (int j = 0; j < k1; j++) hnext[j] *= sc;
         }
       }
       // max_time guard (davidson.h:926-949): stop if the next step would end late
@@ -342,12 +366,15 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
     }
     if (rc) break;
     if (!stop) {
-      // restart with C0 = W (davidson.h:958-962)
+      // restart with C0 = W (davidson.h:958-962); |W| = 1 up to the
+      // orthogonality of the basis, rescaled when it drifts
       double n2 = 0.0;
-      rc = vec_lincomb<VT>(ctx, Cc.data(), last_k, u.data(), spare.p, N, &n2, st);
+      rc = vec_lincomb<VT>(ctx, Cc.data(), last_k, u.data(), 1.0, spare.p, N, &n2, st);
       if (rc) break;
-      rc = vec_scale<VT>(ctx, spare.p, 1.0 / std::sqrt(n2), N, st);
-      if (rc) break;
+      if (std::fabs(n2 - 1.0) > 1e-12) {
+        rc = vec_scale<VT>(ctx, spare.p, 1.0 / std::sqrt(n2), N, st);
+        if (rc) break;
+      }
       std::swap(C[0], spare.p);
       // keep ownership consistent with the buffers
       for (int k = 0; k < nb; k++)
@@ -358,7 +385,7 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
     // W = normalised Ritz vector of the last step
     double n2 = 0.0;
     for (int k = 0; k < last_k; k++) Cc[k] = C[k];
-    rc = vec_lincomb<VT>(ctx, Cc.data(), last_k, u.data(), W, N, &n2, st);
+    rc = vec_lincomb<VT>(ctx, Cc.data(), last_k, u.data(), 1.0, W, N, &n2, st);
     if (!rc) rc = vec_scale<VT>(ctx, W, 1.0 / std::sqrt(n2), N, st);
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = SBD_ERR_DEVICE;
   }
@@ -538,8 +565,8 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
                   ctx->ta.d_rec,  ctx->tb.s_off,  ctx->tb.d_off,  ctx->tb.s_rec,  ctx->tb.d_rec,
                   ctx->d_partials, ctx->d_results, ctx->ta.with_s, ctx->tb.with_s,
                   ctx->ta.ell_ket, ctx->ta.ell_vm, ctx->ta.ell_off,
-                  ctx->tb.ell_ket, ctx->tb.ell_vm, ctx->tb.ell_off,
-                  ctx->ta.ell_col, ctx->tb.ell_col,
+                  ctx->tb.ell_ket, ctx->tb.ell_vm, ctx->tb.ell_off, ctx->d_sdot,
+                  ctx->ta.ell_col, ctx->tb.ell_col, ctx->ta.ell_f32, ctx->tb.ell_f32,
                   ctx->ta.s_bra,   ctx->tb.s_bra,   ctx->d_Ea,       ctx->d_Eb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -752,6 +779,13 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
     if (rc) return rc;
     R->steps_fp32 = R->steps;
     rc = vec_convert<float, double>(ctx, W32.p, W.p, N, nullptr);
+    if (rc) return rc;
+    // renormalise in FP64: the FP32 vector's norm is 1 only to FP32 accuracy,
+    // and the Davidson start vector is taken as normalised
+    const double* Wc[1] = {W.p};
+    double n2 = 0.0;
+    rc = vec_multidot<double>(ctx, Wc, 1, W.p, N, &n2, nullptr);
+    if (!rc && n2 > 0.0) rc = vec_scale<double>(ctx, W.p, 1.0 / std::sqrt(n2), N, nullptr);
     if (rc) return rc;
     SBD_HIP_TRY(ctx, hipDeviceSynchronize());
     (void)hipFree(W32.p);

@@ -112,6 +112,14 @@ struct alignas(16) ValMask {
   uint64_t mask;
 };
 
+// FP32 copy of one doubles slot (ket, value rounded to FP32, sign mask): one
+// 16-byte load in the FP32 sigma.
+struct alignas(16) EllF32 {
+  uint32_t ket;
+  float val;
+  uint64_t mask;
+};
+
 // One spin's excitation tables on the device (CSR by bra string).
 struct SpinTables {
   uint32_t* s_off = nullptr;  // n+1
@@ -132,6 +140,7 @@ struct SpinTables {
   // is odd).
   uint32_t* ell_ket = nullptr;
   ValMask* ell_vm = nullptr;
+  EllF32* ell_f32 = nullptr;  // the same slots for the FP32 sigma
   uint32_t* ell_off = nullptr;  // n_slices + 1
   uint32_t* ell_col = nullptr;  // n: string at each block-sorted position
   uint32_t n_slices = 0;
@@ -171,6 +180,8 @@ struct sbd_ctx {
   double* d_Eb = nullptr;
   uint64_t single_cache_budget = 8ULL << 30;
   // scratch for reductions
+  double* d_sdot = nullptr;  // sigma <T, W> partials (+ result slot)
+  size_t sdot_cap = 0;
   double* d_partials = nullptr;
   double* d_results = nullptr;
   double* h_results = nullptr;  // pinned
@@ -196,8 +207,11 @@ int build_gtable(sbd_ctx* ctx, hipStream_t st);
 int build_single_cache(sbd_ctx* ctx, hipStream_t st);
 int launch_hdiag(sbd_ctx* ctx, double* hii, hipStream_t st);
 int launch_hdiag_f32(sbd_ctx* ctx, float* hii, hipStream_t st);
+// W (+)= H T.  With tw_dot set (and accumulate 0), also returns <T, W> summed in
+// a fixed order (the Davidson diagonal element), synchronising the stream.
 template <typename VT>
-int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st);
+int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st,
+                 double* tw_dot = nullptr);
 
 // Davidson vector kernels; all reductions are deterministic (fixed grid,
 // per-thread Kahan sums, fixed-order final sum) and accumulate in FP64.
@@ -205,6 +219,7 @@ constexpr int kMaxVecs = 40;
 template <typename VT>
 int vec_multidot(sbd_ctx* ctx, const VT* const* X, int m, const VT* y, uint64_t n, double* out,
                  hipStream_t st);
+// out: [0] |r|^2, [1] |w|^2, [2] |t|^2, [3, 3+k) <C_j, t>, [3+k, 3+2k) <HC_j, t>
 template <typename VT>
 int vec_ritz_precond(sbd_ctx* ctx, const VT* const* C, const VT* const* HC, int k, const double* u,
                      double E0, const VT* dii, VT* t, uint64_t n, double* out, hipStream_t st);
@@ -212,8 +227,8 @@ template <typename VT>
 int vec_gs_update(sbd_ctx* ctx, const VT* const* C, int k, const double* coef, VT* t, double scale,
                   uint64_t n, double* out_norm2, hipStream_t st);
 template <typename VT>
-int vec_lincomb(sbd_ctx* ctx, const VT* const* C, int k, const double* u, VT* out, uint64_t n,
-                double* out_norm2, hipStream_t st);
+int vec_lincomb(sbd_ctx* ctx, const VT* const* C, int k, const double* u, double scale, VT* out,
+                uint64_t n, double* out_norm2, hipStream_t st);
 template <typename VT>
 int vec_scale(sbd_ctx* ctx, VT* x, double s, uint64_t n, hipStream_t st);
 template <typename VS, typename VD>

@@ -221,6 +221,7 @@ __global__ void k_single_beta(const uint64_t* __restrict__ a, const uint64_t* __
 }
 
 constexpr int kSigBlock = kColBlock;  // 16 waves: one workgroup per CU when the row fills LDS
+constexpr unsigned kSigMaxGrid = 65536;  // rows beyond this are taken grid-stride
 constexpr int kBatch = 8;        // alpha pairs whose ket loads are issued together
 constexpr int kBBatch = 4;       // beta double slots fetched together
 
@@ -238,21 +239,51 @@ __device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ T
   for (; j < nb; j += kSigBlock) trow[j] = Trow[j];
 }
 
-// One workgroup per bra alpha row (grid-stride over rows); the beta columns
-// are taken kSigBlock at a time.  Each element is accumulated in the
-// reference order, in two phases per block of columns:
-//   1. thread t owns column blk + t: diagonal, alpha singles, alpha doubles
-//      and beta singles; the partial sum is stored to W;
+// Beta-double ELL slot as the kernel of element type VT reads it: FP64 from
+// the ket array and the 16-byte value/mask array, FP32 from one 16-byte record
+// whose value is the FP64 value rounded to FP32 (the FP32 kernel rounds the
+// element to FP32 before its product either way, so both give the same sum).
+template <typename VT>
+struct EllRead;
+template <>
+struct EllRead<double> {
+  __device__ static inline void load(const uint32_t* __restrict__ ket,
+                                     const ValMask* __restrict__ vm, const EllF32* __restrict__,
+                                     size_t s, uint32_t& k, double& v, uint64_t& m) {
+    k = ket[s];
+    const ValMask x = vm[s];
+    v = x.val;
+    m = x.mask;
+  }
+};
+template <>
+struct EllRead<float> {
+  __device__ static inline void load(const uint32_t* __restrict__, const ValMask* __restrict__,
+                                     const EllF32* __restrict__ f, size_t s, uint32_t& k, float& v,
+                                     uint64_t& m) {
+    const EllF32 x = f[s];
+    k = x.ket;
+    v = x.val;
+    m = x.mask;
+  }
+};
+
+// One workgroup per kRows consecutive bra alpha rows (grid-stride over row
+// groups); the beta columns are taken kSigBlock at a time.  Each element is
+// accumulated in the reference order, in two phases per block of columns:
+//   1. thread t owns column blk + t of each row: diagonal, alpha singles,
+//      alpha doubles and beta singles; the partial sum is stored to W;
 //   2. after a barrier, thread t owns the column at block-sorted position
 //      blk + t (the block's columns in descending order of their beta-double
-//      count), reloads its partial sum and adds the beta doubles and the
-//      alpha-single x beta-single terms.
+//      count), reloads its partial sums and adds the beta doubles and the
+//      alpha-single x beta-single terms of every row of the group.
 // Phase 1 keeps every access to T, hii and W coalesced; phase 2 gives each
 // wavefront 64 columns of nearly equal beta-double count, so its ELL slice
-// carries little padding and its lanes finish together.  Every pointer is a
-// restrict-qualified kernel argument: W aliases no input, so the
-// workgroup-uniform alpha records are fetched with scalar loads.
-template <typename VT, typename AT, bool kRowLds, bool kCache>
+// carries little padding and its lanes finish together.  With kRows > 1 (the
+// rows fit LDS together) each beta-double slot is fetched once for all rows.
+// Every pointer is a restrict-qualified kernel argument: W aliases no input,
+// so the workgroup-uniform alpha records are fetched with scalar loads.
+template <typename VT, typename AT, bool kRowLds, bool kCache, int kRows>
 __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint64_t* __restrict__ a_, const uint64_t* __restrict__ b_, uint64_t na_, uint64_t nb_,
     int L_, const VT* __restrict__ hii_, const VT* __restrict__ T_, VT* __restrict__ W_,
@@ -260,89 +291,109 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint32_t* __restrict__ ad_off_, const PairRec* __restrict__ ad_rec_,
     const uint32_t* __restrict__ bs_off_, const PairRec* __restrict__ bs_rec_,
     const uint32_t* __restrict__ bd_off_, const uint32_t* __restrict__ bd_ell_ket_,
-    const ValMask* __restrict__ bd_ell_vm_, const uint32_t* __restrict__ bd_ell_off_,
-    const uint32_t* __restrict__ bd_ell_col_, const double* __restrict__ Ea_,
-    const double* __restrict__ Eb_, uint32_t Sb_, const double* __restrict__ G_,
-    const double* __restrict__ I2_) {
+    const ValMask* __restrict__ bd_ell_vm_, const EllF32* __restrict__ bd_ell_f32_,
+    const uint32_t* __restrict__ bd_ell_off_, const uint32_t* __restrict__ bd_ell_col_,
+    const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
+    const double* __restrict__ G_, const double* __restrict__ I2_,
+    double* __restrict__ dot_part) {
   static_assert(std::is_same<VT, AT>::value, "partial sums round-trip through W");
+  static_assert(kRowLds || kRows == 1, "row groups need the rows in LDS");
   const SigmaArgs<VT> p{a_,      b_,          na_,         nb_,         L_,          hii_,
                         T_,      W_,          accumulate_, as_off_,     as_rec_,     ad_off_,
                         ad_rec_, bs_off_,     bs_rec_,     bd_off_,     bd_ell_ket_, bd_ell_vm_,
                         bd_ell_off_, bd_ell_col_, Ea_,     Eb_,         Sb_,         G_,
                         I2_};
   extern __shared__ unsigned char lds_raw[];
-  VT* trow = reinterpret_cast<VT*>(lds_raw);
+  VT* trows = reinterpret_cast<VT*>(lds_raw);
   const uint32_t nb = (uint32_t)p.nb;
   const int L = p.L;
   const uint32_t nblk = (nb + kSigBlock - 1) / kSigBlock;
-  for (uint64_t ia = blockIdx.x; ia < p.na; ia += gridDim.x) {
-    const VT* __restrict__ Trow = p.T + ia * nb;
-    const VT* __restrict__ Hrow = p.hii + ia * nb;
-    VT* __restrict__ Wrow = p.W + ia * nb;
+  double tw = 0.0;  // sum of T * W over this thread's outputs (when dot_part is set)
+  for (uint64_t ia0 = (uint64_t)blockIdx.x * kRows; ia0 < p.na;
+       ia0 += (uint64_t)gridDim.x * kRows) {
+    const int nr = (int)(p.na - ia0 < (uint64_t)kRows ? p.na - ia0 : kRows);  // uniform
     __syncthreads();
     if (kRowLds) {
-      stage_row<VT>(trow, Trow, nb);
+#pragma unroll
+      for (int r = 0; r < kRows; r++)
+        if (r < nr) stage_row<VT>(trows + (size_t)r * nb, p.T + (ia0 + r) * nb, nb);
       __syncthreads();
     }
-    const VT* __restrict__ Tsrc = kRowLds ? trow : Trow;
-    const uint64_t A = p.a[ia];
-    const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
-    const uint32_t ad0 = p.ad_off[ia], ad1 = p.ad_off[ia + 1];
-    const double* __restrict__ Ebrow = p.Eb ? p.Eb + ia * (uint64_t)p.Sb : nullptr;
+    uint64_t A[kRows];
+    uint32_t as0[kRows], as1[kRows], ad0[kRows], ad1[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; r++) {
+      const uint64_t ia = ia0 + (r < nr ? r : 0);
+      A[r] = p.a[ia];
+      as0[r] = p.as_off[ia];
+      as1[r] = p.as_off[ia + 1];
+      ad0[r] = p.ad_off[ia];
+      ad1[r] = p.ad_off[ia + 1];
+    }
     for (uint32_t g = 0; g < nblk; g++) {
       const uint32_t ib = g * kSigBlock + threadIdx.x;
-      // ---- phase 1: column ib
+      // ---- phase 1: column ib of each row
       if (ib < nb) {
-        const VT hv = Hrow[ib];
         const uint64_t B = p.b[ib];
         const uint32_t bs0 = p.bs_off[ib], bs1 = p.bs_off[ib + 1];
-        const VT ts = Tsrc[ib];
-        AT acc = p.accumulate ? (AT)Wrow[ib] : (AT)0;
-        acc += (AT)hv * (AT)ts;
-        // alpha singles (task type 2, OneExcite): workgroup-uniform records
-        for (uint32_t q = as0; q < as1; q++) {
-          const PairRec r = p.as_rec[q];
-          const VT tk = p.T[(uint64_t)r.ket * nb + ib];
-          const double e =
-              kCache ? p.Ea[(uint64_t)q * nb + ib] : alpha_single_elem(r, A, B, L, p.G);
-          acc += (AT)e * (AT)tk;
-        }
-        // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
-        for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
-          VT tv[kBatch];
-          double vv[kBatch];
-          uint64_t mm[kBatch];
 #pragma unroll
-          for (int u = 0; u < kBatch; u++) {
-            if (q0 + u < ad1) {
-              const PairRec r = p.ad_rec[q0 + u];
-              tv[u] = p.T[(uint64_t)r.ket * nb + ib];
-              vv[u] = r.val;
-              mm[u] = beta_mask_double(r.code);
+        for (int r = 0; r < kRows; r++) {
+          if (r >= nr) continue;
+          const uint64_t ia = ia0 + r;
+          const VT* __restrict__ Tsrc = kRowLds ? trows + (size_t)r * nb : p.T + ia * nb;
+          VT* __restrict__ Wrow = p.W + ia * nb;
+          const VT hv = p.hii[ia * nb + ib];
+          const VT ts = Tsrc[ib];
+          AT acc = p.accumulate ? (AT)Wrow[ib] : (AT)0;
+          acc += (AT)hv * (AT)ts;
+          // alpha singles (task type 2, OneExcite): workgroup-uniform records
+          for (uint32_t q = as0[r]; q < as1[r]; q++) {
+            const PairRec rc = p.as_rec[q];
+            const VT tk = p.T[(uint64_t)rc.ket * nb + ib];
+            const double e = kCache ? p.Ea[(uint64_t)q * nb + ib]
+                                    : alpha_single_elem(rc, A[r], B, L, p.G);
+            acc += (AT)e * (AT)tk;
+          }
+          // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
+          for (uint32_t q0 = ad0[r]; q0 < ad1[r]; q0 += kBatch) {
+            VT tv[kBatch];
+            double vv[kBatch];
+            uint64_t mm[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+              if (q0 + u < ad1[r]) {
+                const PairRec rc = p.ad_rec[q0 + u];
+                tv[u] = p.T[(uint64_t)rc.ket * nb + ib];
+                vv[u] = rc.val;
+                mm[u] = beta_mask_double(rc.code);
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+              if (q0 + u < ad1[r]) {
+                const double e = (__popcll(B & mm[u]) & 1) ? -vv[u] : vv[u];
+                acc += (AT)e * (AT)tv[u];
+              }
             }
           }
-#pragma unroll
-          for (int u = 0; u < kBatch; u++) {
-            if (q0 + u < ad1) {
-              const double e = (__popcll(B & mm[u]) & 1) ? -vv[u] : vv[u];
-              acc += (AT)e * (AT)tv[u];
-            }
+          // beta singles (task type 1, OneExcite): rare, per column
+          for (uint32_t q = bs0; q < bs1; q++) {
+            const PairRec rc = p.bs_rec[q];
+            const double e =
+                kCache ? p.Eb[ia * (uint64_t)p.Sb + q] : beta_single_elem(rc, A[r], B, L, p.G);
+            acc += (AT)e * (AT)Tsrc[rc.ket];
           }
+          Wrow[ib] = (VT)acc;
         }
-        // beta singles (task type 1, OneExcite): rare, per column
-        for (uint32_t q = bs0; q < bs1; q++) {
-          const PairRec r = p.bs_rec[q];
-          const double e = kCache ? Ebrow[q] : beta_single_elem(r, A, B, L, p.G);
-          acc += (AT)e * (AT)Tsrc[r.ket];
-        }
-        Wrow[ib] = (VT)acc;
       }
       __syncthreads();
-      // ---- phase 2: the column at block-sorted position ib
+      // ---- phase 2: the column at block-sorted position ib, every row
       if (ib < nb) {
         const uint32_t c = p.bd_ell_col[ib];
         const uint32_t bd0 = p.bd_off[c], cnt = p.bd_off[c + 1] - bd0;
-        AT acc = (AT)Wrow[c];
+        AT acc[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; r++) acc[r] = r < nr ? (AT)p.W[(ia0 + r) * nb + c] : (AT)0;
         // beta doubles (task type 1, TwoExcite), sliced-ELL: slot k of this
         // position is at base + 64k; slots past the count re-read a valid
         // record and their sum is discarded with a select (bit-identical to
@@ -351,50 +402,89 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
         const size_t base = p.bd_ell_off[sl] + ib % kEllSlice;
         for (uint32_t k0 = 0; k0 < cnt; k0 += kBBatch) {
           uint32_t kk[kBBatch];
-          double vv[kBBatch];
+          VT vv[kBBatch];
           uint64_t mm[kBBatch];
-          VT tt[kBBatch];
 #pragma unroll
           for (int u = 0; u < kBBatch; u++) {
             const uint32_t k = k0 + u < cnt ? k0 + u : cnt - 1;
-            kk[u] = p.bd_ell_ket[base + (size_t)k * kEllSlice];
-            const ValMask vm = p.bd_ell_vm[base + (size_t)k * kEllSlice];
-            vv[u] = vm.val;
-            mm[u] = vm.mask;
+            EllRead<VT>::load(p.bd_ell_ket, p.bd_ell_vm, bd_ell_f32_, base + (size_t)k * kEllSlice,
+                              kk[u], vv[u], mm[u]);
           }
 #pragma unroll
-          for (int u = 0; u < kBBatch; u++) tt[u] = Tsrc[kk[u]];
+          for (int r = 0; r < kRows; r++) {
+            if (r >= nr) continue;
+            const VT* __restrict__ Tsrc =
+                kRowLds ? trows + (size_t)r * nb : p.T + (ia0 + r) * nb;
+            VT tt[kBBatch];
 #pragma unroll
-          for (int u = 0; u < kBBatch; u++) {
-            const double e = (__popcll(A & mm[u]) & 1) ? -vv[u] : vv[u];
-            const AT nv = acc + (AT)e * (AT)tt[u];
-            acc = (k0 + u < cnt) ? nv : acc;
-          }
-        }
-        // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by))
-        if (as1 > as0) {
-          const uint32_t bs0 = p.bs_off[c], bs1 = p.bs_off[c + 1];
-          if (bs1 > bs0) {
-            const uint64_t B = p.b[c];
-            const uint64_t D0 = spread32(A) | (spread32(B) << 1);
-            const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
-            for (uint32_t q = as0; q < as1; q++) {
-              const PairRec ra = p.as_rec[q];
-              const int ai = 2 * code_x0(ra.code), aa = 2 * code_y0(ra.code);
-              const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
-              for (uint32_t k = bs0; k < bs1; k++) {
-                const PairRec rb = p.bs_rec[k];
-                const int bi = 2 * code_x0(rb.code) + 1, ba = 2 * code_y0(rb.code) + 1;
-                const double e = two_excite_full(D0, D1, ai, bi, aa, ba, p.I2);
-                acc += (AT)e * (AT)Tk[rb.ket];
-              }
+            for (int u = 0; u < kBBatch; u++) tt[u] = Tsrc[kk[u]];
+#pragma unroll
+            for (int u = 0; u < kBBatch; u++) {
+              const VT e = (__popcll(A[r] & mm[u]) & 1) ? -vv[u] : vv[u];
+              const AT nv = acc[r] + (AT)e * (AT)tt[u];
+              acc[r] = (k0 + u < cnt) ? nv : acc[r];
             }
           }
         }
-        Wrow[c] = (VT)acc;
+#pragma unroll
+        for (int r = 0; r < kRows; r++) {
+          if (r >= nr) continue;
+          const uint64_t ia = ia0 + r;
+          // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by))
+          if (as1[r] > as0[r]) {
+            const uint32_t bs0 = p.bs_off[c], bs1 = p.bs_off[c + 1];
+            if (bs1 > bs0) {
+              const uint64_t B = p.b[c];
+              const uint64_t D0 = spread32(A[r]) | (spread32(B) << 1);
+              const uint64_t D1 = spread32(A[r] >> 32) | (spread32(B >> 32) << 1);
+              for (uint32_t q = as0[r]; q < as1[r]; q++) {
+                const PairRec ra = p.as_rec[q];
+                const int ai = 2 * code_x0(ra.code), aa = 2 * code_y0(ra.code);
+                const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
+                for (uint32_t k = bs0; k < bs1; k++) {
+                  const PairRec rb = p.bs_rec[k];
+                  const int bi = 2 * code_x0(rb.code) + 1, ba = 2 * code_y0(rb.code) + 1;
+                  const double e = two_excite_full(D0, D1, ai, bi, aa, ba, p.I2);
+                  acc[r] += (AT)e * (AT)Tk[rb.ket];
+                }
+              }
+            }
+          }
+          p.W[ia * nb + c] = (VT)acc[r];
+          if (dot_part) {
+            const VT tc = kRowLds ? trows[(size_t)r * nb + c] : p.T[ia * nb + c];
+            tw += (double)tc * (double)(VT)acc[r];
+          }
+        }
       }
     }
   }
+  if (dot_part) {
+    for (int o = 32; o > 0; o >>= 1) tw += __shfl_down(tw, o, 64);
+    if ((threadIdx.x & 63) == 0) dot_part[blockIdx.x * (kSigBlock / 64) + threadIdx.x / 64] = tw;
+  }
+}
+
+// Sum of n partials in a fixed order: thread t adds partials t, t + 1024, ...
+// (compensated), then the 1024 thread sums are combined in a fixed tree.
+constexpr int kSumBlock = 1024;
+__global__ __launch_bounds__(kSumBlock) void k_sum_fixed(const double* __restrict__ part,
+                                                         uint64_t n, double* __restrict__ out) {
+  __shared__ double red[kSumBlock];
+  double s = 0.0, c = 0.0;
+  for (uint64_t i = threadIdx.x; i < n; i += kSumBlock) {
+    const double y = part[i] - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kSumBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
 }
 
 int build_single_cache(sbd_ctx* ctx, hipStream_t st) {
@@ -422,7 +512,8 @@ int build_single_cache(sbd_ctx* ctx, hipStream_t st) {
 }
 
 template <typename VT>
-int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st) {
+int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st,
+                 double* tw_dot) {
   if (!ctx->tables_built) {
     ctx->last_error = "sigma called before sbd_make_helpers";
     return SBD_ERR_STATE;
@@ -439,47 +530,76 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
   }
   const size_t row_bytes = sizeof(VT) * ctx->nb;
   const bool lds = row_bytes <= (size_t)ctx->lds_max;
-  const unsigned grid = (unsigned)(ctx->na < 0x7FFFFFFFULL ? ctx->na : 0x7FFFFFFFULL);
-  const size_t shm = lds ? row_bytes : 0;
+  // rows per workgroup: two when both fit LDS together (and there are two)
+  const int rows = (lds && 2 * row_bytes <= (size_t)ctx->lds_max && ctx->na >= 2) ? 2 : 1;
+  const uint64_t groups = (ctx->na + rows - 1) / rows;
+  const unsigned grid = (unsigned)(groups < (uint64_t)kSigMaxGrid ? groups : kSigMaxGrid);
+  double* dpart = nullptr;
+  if (tw_dot) {
+    const size_t need = (size_t)grid * (kSigBlock / 64) + 1;
+    if (ctx->sdot_cap < need) {
+      if (ctx->d_sdot) (void)hipFree(ctx->d_sdot);
+      ctx->d_sdot = nullptr;
+      ctx->sdot_cap = 0;
+      SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_sdot, sizeof(double) * need));
+      ctx->sdot_cap = need;
+    }
+    dpart = ctx->d_sdot;
+  }
+  const size_t shm = lds ? rows * row_bytes : 0;
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
-#define SBD_SIGMA_ARGS                                                                         \
-  ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,    \
-      A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_vm,           \
-      B.ell_off, B.ell_col, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,         \
-      (const double*)ctx->d_G, (const double*)ctx->d_eri
+#define SBD_SIGMA_ARGS                                                                          \
+  ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,     \
+      A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_vm, B.ell_f32, B.ell_off,   \
+      B.ell_col, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,           \
+      (const double*)ctx->d_G, (const double*)ctx->d_eri, dpart
+#define SBD_LAUNCH(LDS, CACHE, ROWS)                                                            \
+  hipLaunchKernelGGL((k_sigma_rows<VT, VT, LDS, CACHE, ROWS>), dim3(grid), dim3(kSigBlock), shm, \
+                     st, SBD_SIGMA_ARGS)
   // precomputed single-excitation elements, when both caches exist
   const bool cache = (A.n_s == 0 || ctx->d_Ea) && (B.n_s == 0 || ctx->d_Eb);
-  if (lds && cache)
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, true, true>), dim3(grid), dim3(kSigBlock), shm, st,
-                       SBD_SIGMA_ARGS);
-  else if (lds)
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, true, false>), dim3(grid), dim3(kSigBlock), shm, st,
-                       SBD_SIGMA_ARGS);
-  else if (cache)
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, false, true>), dim3(grid), dim3(kSigBlock), 0, st,
-                       SBD_SIGMA_ARGS);
-  else
-    hipLaunchKernelGGL((k_sigma_rows<VT, VT, false, false>), dim3(grid), dim3(kSigBlock), 0, st,
-                       SBD_SIGMA_ARGS);
+  if (lds && rows == 2) {
+    if (cache) SBD_LAUNCH(true, true, 2);
+    else SBD_LAUNCH(true, false, 2);
+  } else if (lds) {
+    if (cache) SBD_LAUNCH(true, true, 1);
+    else SBD_LAUNCH(true, false, 1);
+  } else {
+    if (cache) SBD_LAUNCH(false, true, 1);
+    else SBD_LAUNCH(false, false, 1);
+  }
+#undef SBD_LAUNCH
 #undef SBD_SIGMA_ARGS
   SBD_HIP_TRY(ctx, hipGetLastError());
+  if (tw_dot) {
+    const uint64_t np = (uint64_t)grid * (kSigBlock / 64);
+    hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kSumBlock), 0, st, dpart, np, dpart + np);
+    SBD_HIP_TRY(ctx, hipGetLastError());
+    SBD_HIP_TRY(ctx, hipMemcpyAsync(tw_dot, dpart + np, sizeof(double), hipMemcpyDeviceToHost, st));
+    SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
+  }
   return SBD_OK;
 }
 
 template int launch_sigma<double>(sbd_ctx*, const double*, const double*, double*, int,
-                                  hipStream_t);
-template int launch_sigma<float>(sbd_ctx*, const float*, const float*, float*, int, hipStream_t);
+                                  hipStream_t, double*);
+template int launch_sigma<float>(sbd_ctx*, const float*, const float*, float*, int, hipStream_t,
+                                 double*);
 
 int sigma_set_lds_limit(sbd_ctx* ctx) {
   // allow the row-staged variants to use the device's full LDS
   int lds = 65536;
   SBD_HIP_TRY(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                          ctx->device));
-  const void* fns[] = {(const void*)k_sigma_rows<double, double, true, true>,
-                       (const void*)k_sigma_rows<double, double, true, false>,
-                       (const void*)k_sigma_rows<float, float, true, true>,
-                       (const void*)k_sigma_rows<float, float, true, false>};
+  const void* fns[] = {(const void*)k_sigma_rows<double, double, true, true, 1>,
+                       (const void*)k_sigma_rows<double, double, true, false, 1>,
+                       (const void*)k_sigma_rows<double, double, true, true, 2>,
+                       (const void*)k_sigma_rows<double, double, true, false, 2>,
+                       (const void*)k_sigma_rows<float, float, true, true, 1>,
+                       (const void*)k_sigma_rows<float, float, true, false, 1>,
+                       (const void*)k_sigma_rows<float, float, true, true, 2>,
+                       (const void*)k_sigma_rows<float, float, true, false, 2>};
   for (const void* f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipGetLastError();

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kTabBlock) void k_tab_fill(const uint64_t* __restri
 __global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off,
                           const PairRec* __restrict__ rec, const uint32_t* __restrict__ ell_off,
                           const uint32_t* __restrict__ ell_col, uint32_t* __restrict__ ell_ket,
-                          ValMask* __restrict__ ell_vm) {
+                          ValMask* __restrict__ ell_vm, EllF32* __restrict__ ell_f32) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = i / kEllSlice, l = i % kEllSlice;
@@ -139,6 +139,7 @@ __global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off
     const size_t slot = ell_off[s] + (size_t)u * kEllSlice + l;
     ell_ket[slot] = ket;
     ell_vm[slot] = ValMask{val, mask};
+    ell_f32[slot] = EllF32{ket, (float)val, mask};
   }
 }
 
@@ -245,8 +246,9 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
   const size_t nslot = (size_t)eoff[T.n_slices] + 1;
   SBD_HIP_TRY(ctx, hipMalloc(&T.ell_ket, sizeof(uint32_t) * nslot));
   SBD_HIP_TRY(ctx, hipMalloc(&T.ell_vm, sizeof(ValMask) * nslot));
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_f32, sizeof(EllF32) * nslot));
   hipLaunchKernelGGL(k_tab_ell, dim3((n + 255) / 256), dim3(256), 0, st, n, spin, T.d_off,
-                     T.d_rec, T.ell_off, T.ell_col, T.ell_ket, T.ell_vm);
+                     T.d_rec, T.ell_off, T.ell_col, T.ell_ket, T.ell_vm, T.ell_f32);
   SBD_HIP_TRY(ctx, hipGetLastError());
   // bra string of every single record
   std::vector<uint32_t> sb(tot_s + 1, 0);

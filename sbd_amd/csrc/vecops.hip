@@ -11,6 +11,8 @@
 // All sums are FP64 with per-thread Kahan compensation; every kernel runs a
 // fixed grid and the block partials are summed in a fixed order, so results
 // are deterministic run to run.
+#include <initializer_list>
+
 #include "sbd_internal.hpp"
 
 namespace sbdx {
@@ -55,112 +57,223 @@ __device__ inline void block_partials(double (&v)[MAXM], int m, double* __restri
   }
 }
 
-__global__ void k_finalize(const double* __restrict__ part, int nblk, int m,
-                           double* __restrict__ out) {
-  const int j = threadIdx.x;
-  if (j >= m) return;
+// One block per sum: thread t adds block partials t, t + 256, ... (compensated),
+// then the 256 thread sums are combined in a fixed tree.
+__global__ __launch_bounds__(kVecBlock) void k_finalize(const double* __restrict__ part, int nblk,
+                                                        int m, double* __restrict__ out) {
+  __shared__ double red[kVecBlock];
+  const int j = blockIdx.x;
   Kahan k;
-  for (int b = 0; b < nblk; b++) k.add(part[b * m + j]);
-  out[j] = k.s;
+  for (int b = threadIdx.x; b < nblk; b += kVecBlock) k.add(part[b * m + j]);
+  red[threadIdx.x] = k.s;
+  __syncthreads();
+  for (int w = kVecBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// Elements are read as 16-byte chunks (two doubles or four floats) when every
+// vector of the call is 16-byte aligned; the remainder (and any call with an
+// unaligned vector) goes element by element.  Each thread owns a fixed set of
+// chunks, so the per-thread sums, and hence the results, are the same on
+// every run.
+template <typename VT, int W>
+struct alignas(W * sizeof(VT)) Chunk {
+  VT v[W];
+};
+template <int W, typename VT>
+__device__ inline Chunk<VT, W> ldc(const VT* __restrict__ p, uint64_t i) {
+  return *reinterpret_cast<const Chunk<VT, W>*>(p + i);
+}
+template <int W, typename VT>
+__device__ inline void stc(VT* __restrict__ p, uint64_t i, const Chunk<VT, W>& c) {
+  *reinterpret_cast<Chunk<VT, W>*>(p + i) = c;
+}
+template <typename VT>
+constexpr int chunk_of() {
+  return 16 / (int)sizeof(VT);  // 16-byte loads: two doubles or four floats
+}
+
+// Calls body.template operator()<W>(i) over this thread's share of [0, n):
+// W-wide chunks at i = W * (tid + j * nthreads) while they fit (if packed),
+// then single elements.
+template <typename VT, typename F, int P = chunk_of<VT>()>
+__device__ inline void for_chunks(uint64_t n, bool packed, F&& body) {
+  const uint64_t tid = blockIdx.x * (uint64_t)kVecBlock + threadIdx.x;
+  const uint64_t nth = (uint64_t)gridDim.x * kVecBlock;
+  const uint64_t np = packed ? n / P : 0;
+  for (uint64_t q = tid; q < np; q += nth) body.template operator()<P>(q * P);
+  for (uint64_t i = np * P + tid; i < n; i += nth) body.template operator()<1>(i);
 }
 
 template <typename VT, int MAXM>
 __global__ __launch_bounds__(kVecBlock) void k_multidot(VecPtrs<VT> X, int m,
                                                         const VT* __restrict__ y, uint64_t n,
-                                                        double* __restrict__ part) {
+                                                        bool packed, double* __restrict__ part) {
   Kahan acc[MAXM];
-  for (uint64_t i = blockIdx.x * (uint64_t)kVecBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kVecBlock) {
-    const double yi = (double)y[i];
+  for_chunks<VT>(n, packed, [&]<int W>(uint64_t i) {
+    const Chunk<VT, W> yv = ldc<W>(y, i);
 #pragma unroll
-    for (int j = 0; j < MAXM; j++)
-      if (j < m) acc[j].add((double)X.p[j][i] * yi);
-  }
+    for (int j = 0; j < MAXM; j++) {
+      if (j < m) {
+        const Chunk<VT, W> xv = ldc<W>(X.p[j], i);
+#pragma unroll
+        for (int e = 0; e < W; e++) acc[j].add((double)xv.v[e] * (double)yv.v[e]);
+      }
+    }
+  });
   double v[MAXM];
 #pragma unroll
   for (int j = 0; j < MAXM; j++) v[j] = acc[j].s;
   block_partials<MAXM>(v, m, part);
 }
 
-// sums: [0] |r|^2, [1] |w|^2, [2] |t|^2, [3 .. 3+k) <C_j, t>
+// One pass of the Davidson step after the Ritz problem:
+//   w = sum u_j C_j,  r = -(sum u_j HC_j) + E0 w,  t = r / (E0 - d)
+// with the 1e-12 guard of the reference preconditioner (davidson.h:868-877).
+// sums: [0] |r|^2, [1] |w|^2, [2] |t|^2, [3, 3+k) <C_j, t>, [3+MAXK, 3+MAXK+k)
+// <HC_j, t> (t as stored); the last group gives the next projected-matrix
+// column by symmetry of H, so no separate overlap pass follows the sigma.
 template <typename VT, int MAXK>
 __global__ __launch_bounds__(kVecBlock) void k_ritz_precond(VecPtrs<VT> C, VecPtrs<VT> HC, int k,
                                                             Coefs u, double E0,
                                                             const VT* __restrict__ dii,
                                                             VT* __restrict__ t, uint64_t n,
+                                                            bool packed,
                                                             double* __restrict__ part) {
   constexpr double eps_reg = 1.0e-12;
-  Kahan rr, ww, tt, ot[MAXK];
-  for (uint64_t i = blockIdx.x * (uint64_t)kVecBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kVecBlock) {
-    double c[MAXK];
+  // norms compensated; the 2k overlaps as plain FP64 per-thread sums (the
+  // register budget of 2k compensated sums would cut the waves in flight)
+  Kahan rr, ww, tt;
+  double ot[MAXK], oh[MAXK];
 #pragma unroll
-    for (int j = 0; j < MAXK; j++) c[j] = j < k ? (double)C.p[j][i] : 0.0;
-    double w = c[0] * u.c[0];
-    double r = (double)HC.p[0][i] * (-1.0 * u.c[0]);
+  for (int j = 0; j < MAXK; j++) ot[j] = oh[j] = 0.0;
+  // all 2k chunks are held until t is known: above 8 vectors FP32 takes
+  // 8-byte chunks so that the registers leave room for a few waves per SIMD
+  auto body = [&]<int W>(uint64_t i) {
+    double w[W], r[W];
+    Chunk<VT, W> cv[MAXK], hv[MAXK];
+#pragma unroll
+    for (int j = 0; j < MAXK; j++) {
+      if (j < k) {
+        cv[j] = ldc<W>(C.p[j], i);
+        hv[j] = ldc<W>(HC.p[j], i);
+      }
+    }
+    const Chunk<VT, W> dv = ldc<W>(dii, i);
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      w[e] = (double)cv[0].v[e] * u.c[0];
+      r[e] = (double)hv[0].v[e] * (-1.0 * u.c[0]);
+    }
 #pragma unroll
     for (int j = 1; j < MAXK; j++) {
       if (j < k) {
-        w += c[j] * u.c[j];
-        r += (double)HC.p[j][i] * (-1.0 * u.c[j]);
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+          w[e] += (double)cv[j].v[e] * u.c[j];
+          r[e] += (double)hv[j].v[e] * (-1.0 * u.c[j]);
+        }
       }
     }
-    r += E0 * w;
-    const double den = E0 - (double)dii[i];
-    double tv = fabs(den) > eps_reg ? r / den : r / (den - eps_reg);
-    const VT ts = (VT)tv;
-    t[i] = ts;
-    tv = (double)ts;
-    rr.add(r * r);
-    ww.add(w * w);
-    tt.add(tv * tv);
+    Chunk<VT, W> ts;
 #pragma unroll
-    for (int j = 0; j < MAXK; j++)
-      if (j < k) ot[j].add(c[j] * tv);
-  }
-  double v[MAXK + 3];
+    for (int e = 0; e < W; e++) {
+      r[e] += E0 * w[e];
+      const double den = E0 - (double)dv.v[e];
+      const double tv = fabs(den) > eps_reg ? r[e] / den : r[e] / (den - eps_reg);
+      ts.v[e] = (VT)tv;
+    }
+    stc<W>(t, i, ts);
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      const double tv = (double)ts.v[e];
+      rr.add(r[e] * r[e]);
+      ww.add(w[e] * w[e]);
+      tt.add(tv * tv);
+#pragma unroll
+      for (int j = 0; j < MAXK; j++) {
+        if (j < k) {
+          ot[j] += (double)cv[j].v[e] * tv;
+          oh[j] += (double)hv[j].v[e] * tv;
+        }
+      }
+    }
+  };
+  constexpr int P = (sizeof(VT) == 4 && MAXK > 8) ? 2 : chunk_of<VT>();
+  for_chunks<VT, decltype(body)&, P>(n, packed, body);
+  double v[2 * MAXK + 3];
   v[0] = rr.s;
   v[1] = ww.s;
   v[2] = tt.s;
 #pragma unroll
-  for (int j = 0; j < MAXK; j++) v[3 + j] = ot[j].s;
-  block_partials<MAXK + 3>(v, k + 3, part);
+  for (int j = 0; j < MAXK; j++) {
+    v[3 + j] = ot[j];
+    v[3 + MAXK + j] = oh[j];
+  }
+  block_partials<2 * MAXK + 3>(v, 2 * MAXK + 3, part);
 }
 
+// Classical Gram-Schmidt update t = scale * (t - sum o_j C_j); sums |t|^2.
 template <typename VT, int MAXK>
 __global__ __launch_bounds__(kVecBlock) void k_gs_update(VecPtrs<VT> C, int k, Coefs o,
                                                          VT* __restrict__ t, double scale,
-                                                         uint64_t n, double* __restrict__ part) {
+                                                         uint64_t n, bool packed,
+                                                         double* __restrict__ part) {
   Kahan tt;
-  for (uint64_t i = blockIdx.x * (uint64_t)kVecBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kVecBlock) {
-    double x = (double)t[i];
+  for_chunks<VT>(n, packed, [&]<int W>(uint64_t i) {
+    const Chunk<VT, W> tv = ldc<W>(t, i);
+    double x[W];
 #pragma unroll
-    for (int j = 0; j < MAXK; j++)
-      if (j < k) x -= o.c[j] * (double)C.p[j][i];
-    const VT xs = (VT)(x * scale);
-    t[i] = xs;
-    tt.add((double)xs * (double)xs);
-  }
+    for (int e = 0; e < W; e++) x[e] = (double)tv.v[e];
+#pragma unroll
+    for (int j = 0; j < MAXK; j++) {
+      if (j < k) {
+        const Chunk<VT, W> cv = ldc<W>(C.p[j], i);
+#pragma unroll
+        for (int e = 0; e < W; e++) x[e] -= o.c[j] * (double)cv.v[e];
+      }
+    }
+    Chunk<VT, W> xs;
+#pragma unroll
+    for (int e = 0; e < W; e++) xs.v[e] = (VT)(x[e] * scale);
+    stc<W>(t, i, xs);
+#pragma unroll
+    for (int e = 0; e < W; e++) tt.add((double)xs.v[e] * (double)xs.v[e]);
+  });
   double v[1] = {tt.s};
   block_partials<1>(v, 1, part);
 }
 
+// out = scale * sum u_j C_j; sums |out|^2.
 template <typename VT, int MAXK>
 __global__ __launch_bounds__(kVecBlock) void k_lincomb(VecPtrs<VT> C, int k, Coefs u,
-                                                       VT* __restrict__ out, uint64_t n,
+                                                       VT* __restrict__ out, double scale,
+                                                       uint64_t n, bool packed,
                                                        double* __restrict__ part) {
   Kahan tt;
-  for (uint64_t i = blockIdx.x * (uint64_t)kVecBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kVecBlock) {
-    double x = (double)C.p[0][i] * u.c[0];
+  for_chunks<VT>(n, packed, [&]<int W>(uint64_t i) {
+    double x[W];
+    const Chunk<VT, W> c0 = ldc<W>(C.p[0], i);
 #pragma unroll
-    for (int j = 1; j < MAXK; j++)
-      if (j < k) x += (double)C.p[j][i] * u.c[j];
-    const VT xs = (VT)x;
-    out[i] = xs;
-    tt.add((double)xs * (double)xs);
-  }
+    for (int e = 0; e < W; e++) x[e] = (double)c0.v[e] * u.c[0];
+#pragma unroll
+    for (int j = 1; j < MAXK; j++) {
+      if (j < k) {
+        const Chunk<VT, W> cv = ldc<W>(C.p[j], i);
+#pragma unroll
+        for (int e = 0; e < W; e++) x[e] += (double)cv.v[e] * u.c[j];
+      }
+    }
+    Chunk<VT, W> xs;
+#pragma unroll
+    for (int e = 0; e < W; e++) xs.v[e] = (VT)(x[e] * scale);
+    stc<W>(out, i, xs);
+#pragma unroll
+    for (int e = 0; e < W; e++) tt.add((double)xs.v[e] * (double)xs.v[e]);
+  });
   double v[1] = {tt.s};
   block_partials<1>(v, 1, part);
 }
@@ -201,7 +314,7 @@ static int grid_for(uint64_t n) {
 }
 
 static int finish(sbd_ctx* ctx, int grid, int m, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64 * ((m + 63) / 64)), 0, st, ctx->d_partials, grid,
+  hipLaunchKernelGGL(k_finalize, dim3(m), dim3(kVecBlock), 0, st, ctx->d_partials, grid,
                      m, ctx->d_results);
   SBD_HIP_TRY(ctx, hipGetLastError());
   SBD_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_results, ctx->d_results, sizeof(double) * m,
@@ -224,15 +337,35 @@ static Coefs packc(const double* c, int m) {
   return C;
 }
 
-// dispatch a kernel template over compile-time sizes 4/8/16/24/40
+// compile-time sizes the kernels are instantiated for
+static int size_class(int k) {
+  return k <= 4 ? 4 : k <= 8 ? 8 : k <= 12 ? 12 : k <= 16 ? 16 : k <= 24 ? 24 : kMaxVecs;
+}
 #define SBD_DISPATCH_K(k, KERN, ...)                                                       \
   do {                                                                                     \
-    if ((k) <= 4) hipLaunchKernelGGL((KERN<VT, 4>), __VA_ARGS__);                          \
-    else if ((k) <= 8) hipLaunchKernelGGL((KERN<VT, 8>), __VA_ARGS__);                     \
-    else if ((k) <= 16) hipLaunchKernelGGL((KERN<VT, 16>), __VA_ARGS__);                   \
-    else if ((k) <= 24) hipLaunchKernelGGL((KERN<VT, 24>), __VA_ARGS__);                   \
-    else hipLaunchKernelGGL((KERN<VT, kMaxVecs>), __VA_ARGS__);                            \
+    switch (size_class(k)) {                                                               \
+      case 4: hipLaunchKernelGGL((KERN<VT, 4>), __VA_ARGS__); break;                       \
+      case 8: hipLaunchKernelGGL((KERN<VT, 8>), __VA_ARGS__); break;                       \
+      case 12: hipLaunchKernelGGL((KERN<VT, 12>), __VA_ARGS__); break;                     \
+      case 16: hipLaunchKernelGGL((KERN<VT, 16>), __VA_ARGS__); break;                     \
+      case 24: hipLaunchKernelGGL((KERN<VT, 24>), __VA_ARGS__); break;                     \
+      default: hipLaunchKernelGGL((KERN<VT, kMaxVecs>), __VA_ARGS__); break;               \
+    }                                                                                      \
   } while (0)
+
+// true when every pointer is 16-byte aligned (chunked loads allowed)
+template <typename VT>
+static bool aligned16(std::initializer_list<const VT*> ps) {
+  for (const VT* p : ps)
+    if (reinterpret_cast<uintptr_t>(p) % 16) return false;
+  return true;
+}
+template <typename VT>
+static bool aligned16(const VT* const* X, int m) {
+  for (int j = 0; j < m; j++)
+    if (reinterpret_cast<uintptr_t>(X[j]) % 16) return false;
+  return true;
+}
 
 template <typename VT>
 int vec_multidot(sbd_ctx* ctx, const VT* const* X, int m, const VT* y, uint64_t n, double* out,
@@ -241,7 +374,8 @@ int vec_multidot(sbd_ctx* ctx, const VT* const* X, int m, const VT* y, uint64_t 
   int rc = ensure_scratch(ctx);
   if (rc) return rc;
   const int g = grid_for(n);
-  SBD_DISPATCH_K(m, k_multidot, dim3(g), dim3(kVecBlock), 0, st, pack(X, m), m, y, n,
+  const bool packed = aligned16(X, m) && aligned16<VT>({y});
+  SBD_DISPATCH_K(m, k_multidot, dim3(g), dim3(kVecBlock), 0, st, pack(X, m), m, y, n, packed,
                  ctx->d_partials);
   SBD_HIP_TRY(ctx, hipGetLastError());
   return finish(ctx, g, m, out, st);
@@ -254,10 +388,17 @@ int vec_ritz_precond(sbd_ctx* ctx, const VT* const* C, const VT* const* HC, int 
   int rc = ensure_scratch(ctx);
   if (rc) return rc;
   const int g = grid_for(n);
+  const bool packed = aligned16(C, k) && aligned16(HC, k) && aligned16<VT>({dii, t});
   SBD_DISPATCH_K(k, k_ritz_precond, dim3(g), dim3(kVecBlock), 0, st, pack(C, k), pack(HC, k), k,
-                 packc(u, k), E0, dii, t, n, ctx->d_partials);
+                 packc(u, k), E0, dii, t, n, packed, ctx->d_partials);
   SBD_HIP_TRY(ctx, hipGetLastError());
-  return finish(ctx, g, k + 3, out, st);
+  const int K = size_class(k);
+  std::vector<double> res(2 * K + 3);
+  rc = finish(ctx, g, 2 * K + 3, res.data(), st);
+  if (rc) return rc;
+  for (int j = 0; j < 3 + k; j++) out[j] = res[j];
+  for (int j = 0; j < k; j++) out[3 + k + j] = res[3 + K + j];
+  return SBD_OK;
 }
 
 template <typename VT>
@@ -267,21 +408,23 @@ int vec_gs_update(sbd_ctx* ctx, const VT* const* C, int k, const double* coef, V
   int rc = ensure_scratch(ctx);
   if (rc) return rc;
   const int g = grid_for(n);
+  const bool packed = aligned16(C, k) && aligned16<VT>({t});
   SBD_DISPATCH_K(k, k_gs_update, dim3(g), dim3(kVecBlock), 0, st, pack(C, k), k, packc(coef, k), t,
-                 scale, n, ctx->d_partials);
+                 scale, n, packed, ctx->d_partials);
   SBD_HIP_TRY(ctx, hipGetLastError());
   return finish(ctx, g, 1, out_norm2, st);
 }
 
 template <typename VT>
-int vec_lincomb(sbd_ctx* ctx, const VT* const* C, int k, const double* u, VT* out, uint64_t n,
-                double* out_norm2, hipStream_t st) {
+int vec_lincomb(sbd_ctx* ctx, const VT* const* C, int k, const double* u, double scale, VT* out,
+                uint64_t n, double* out_norm2, hipStream_t st) {
   if (k < 1 || k > kMaxVecs) return SBD_ERR_ARG;
   int rc = ensure_scratch(ctx);
   if (rc) return rc;
   const int g = grid_for(n);
-  SBD_DISPATCH_K(k, k_lincomb, dim3(g), dim3(kVecBlock), 0, st, pack(C, k), k, packc(u, k), out, n,
-                 ctx->d_partials);
+  const bool packed = aligned16(C, k) && aligned16<VT>({out});
+  SBD_DISPATCH_K(k, k_lincomb, dim3(g), dim3(kVecBlock), 0, st, pack(C, k), k, packc(u, k), out,
+                 scale, n, packed, ctx->d_partials);
   SBD_HIP_TRY(ctx, hipGetLastError());
   return finish(ctx, g, 1, out_norm2, st);
 }
@@ -308,8 +451,8 @@ int vec_convert(sbd_ctx* ctx, const VS* x, VD* y, uint64_t n, hipStream_t st) {
                                     hipStream_t);                                             \
   template int vec_gs_update<VT>(sbd_ctx*, const VT* const*, int, const double*, VT*, double,  \
                                  uint64_t, double*, hipStream_t);                             \
-  template int vec_lincomb<VT>(sbd_ctx*, const VT* const*, int, const double*, VT*, uint64_t,  \
-                               double*, hipStream_t);                                         \
+  template int vec_lincomb<VT>(sbd_ctx*, const VT* const*, int, const double*, double, VT*,    \
+                               uint64_t, double*, hipStream_t);                               \
   template int vec_scale<VT>(sbd_ctx*, VT*, double, uint64_t, hipStream_t);
 SBD_INST(double)
 SBD_INST(float)
