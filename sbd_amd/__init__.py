@@ -237,7 +237,10 @@ class MultTPB:
             self._ctx = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: module globals may be gone
+            pass
 
     def make_helpers(self):
         _check(lib().sbd_make_helpers(self._ctx), self._ctx)

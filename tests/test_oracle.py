@@ -127,6 +127,26 @@ def test_syev_matches_numpy(O):
         assert np.abs(U.T @ U - np.eye(n)).max() < 1e-12
 
 
+@pytest.mark.parametrize("L,na,nb,seed", [(6, 3, 3, 19), (7, 3, 2, 23)])
+def test_oracle_davidson_reaches_dense_ground_state(O, L, na, nb, seed):
+    """The oracle's Davidson (davidson.h:707-970 restated) converges to the lowest
+    eigenvalue of the dense H built independently by second quantisation, and its
+    vector is that eigenvector (|overlap| = 1)."""
+    from oracle import brute
+    from sbd_amd import synth
+    _, h, eri = synth.make_integrals(L, seed, 2 * L)
+    a, b = synth.all_strings(L, na), synth.all_strings(L, nb)
+    S = O.System.from_workload(dict(norb=L, I0=0.5, h=h, eri=eri, a=a, b=b))
+    hii = S.hdiag()
+    W0 = synth.random_init_vector(S.N, 31)
+    W, hist = S.davidson(hii, W0, 200, 10, 1e-9)
+    assert hist[-1][1] < 1e-9
+    Hb = brute.dense_hamiltonian(L, 0.5, h, synth.unpack_eri(eri, L), a, b)
+    ev, V = np.linalg.eigh(Hb)
+    assert abs(S.energy(hii, W) - ev[0]) < 1e-10 * max(1.0, abs(ev[0]))
+    assert abs(abs(np.dot(V[:, 0], W)) - 1.0) < 1e-9
+
+
 @pytest.mark.parametrize("mol,alpha,E_known", [
     ("h2o", "h2o-1em3-alpha.txt.gz", -76.23594663),
     ("n2", "1em3-alpha.txt.gz", -109.04162110),
