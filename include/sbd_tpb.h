@@ -177,6 +177,50 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
  * integrals), for admission control. */
 int sbd_diag_footprint(const sbd_ctx* ctx, const sbd_options* opt, uint64_t* bytes);
 
+/* ------------------------------------------------------------------ serving
+ * A server runs many independent solves on the current GPU with continuous
+ * batching: requests are admitted as device memory (mem_budget bytes) and
+ * slots (max_active) allow, every admitted solve advances one Davidson step
+ * per engine round, and each kind of device pass runs for all of them in one
+ * launch.  A solve's result is bit-identical alone, in any mix, and to
+ * sbd_diag with the same options.  The reference has no serving mode; the
+ * solves are sbd::tpb::diag (chemistry/tpb/sbdiag.h:151-526, method 0, FP64).
+ * Server calls are thread-safe. */
+typedef struct sbd_server sbd_server;
+
+typedef struct {
+  int norb;
+  double I0;
+  const double* h1;         /* norb x norb one-body integrals, row-major */
+  const double* eri_packed; /* B(B+1)/2, B = norb(norb+1)/2 */
+  const uint64_t* a_strings;
+  size_t na;
+  const uint64_t* b_strings;
+  size_t nb;
+  sbd_options opt; /* method 0, precision 0; max_it, max_nb, eps, init, seed */
+} sbd_request;
+
+typedef struct {
+  double energy;   /* <W, H W> of the final vector, as sbd_diag */
+  double residual; /* |H W - E0 W| of the last step */
+  int converged;
+  int steps;
+  int status;      /* sbd_status of the solve */
+  double t_submit, t_admit, t_done; /* seconds on the server clock */
+} sbd_result;
+
+sbd_server* sbd_server_create(uint64_t mem_budget, int max_active, int* status);
+/* Copies the request's inputs and queues it; returns at once. */
+int sbd_server_submit(sbd_server* sv, const sbd_request* rq, uint64_t* id);
+/* Blocks until the solve id has finished. */
+int sbd_server_wait(sbd_server* sv, uint64_t id, sbd_result* out);
+int sbd_server_poll(sbd_server* sv, uint64_t id, sbd_result* out, int* done);
+double sbd_server_now(const sbd_server* sv);
+/* out[0] = engine rounds so far, out[1] = most solves active in one round. */
+int sbd_server_stats(const sbd_server* sv, uint64_t* out);
+/* Finishes every queued solve, then frees the server. */
+void sbd_server_destroy(sbd_server* sv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,9 @@ EXPORTS = [
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
     "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
-    "sbd_diag", "sbd_diag_footprint",
+    "sbd_diag", "sbd_diag_footprint", "sbd_server_create", "sbd_server_submit",
+    "sbd_server_wait", "sbd_server_poll", "sbd_server_now", "sbd_server_stats",
+    "sbd_server_destroy",
 ]
 
 
@@ -51,6 +53,22 @@ class Report(C.Structure):
     _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("energy", C.c_double), ("t_tables", C.c_double), ("t_hdiag", C.c_double),
                 ("t_davidson", C.c_double), ("t_sigma", C.c_double), ("steps_fp32", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Request(C.Structure):
+    _fields_ = [("norb", C.c_int), ("I0", C.c_double), ("h1", C.POINTER(C.c_double)),
+                ("eri_packed", C.POINTER(C.c_double)), ("a_strings", C.POINTER(C.c_uint64)),
+                ("na", C.c_size_t), ("b_strings", C.POINTER(C.c_uint64)), ("nb", C.c_size_t),
+                ("opt", Options)]
+
+
+class Result(C.Structure):
+    _fields_ = [("energy", C.c_double), ("residual", C.c_double), ("converged", C.c_int),
+                ("steps", C.c_int), ("status", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -93,6 +111,15 @@ def lib():
     L.sbd_davidson.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_diag.argtypes = [vp, C.POINTER(Options), dp, dp, dp, C.c_int, C.POINTER(Report)]
     L.sbd_diag_footprint.argtypes = [vp, C.POINTER(Options), up]
+    L.sbd_server_create.restype = vp
+    L.sbd_server_create.argtypes = [C.c_uint64, C.c_int, ip]
+    L.sbd_server_submit.argtypes = [vp, C.POINTER(Request), up]
+    L.sbd_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(Result)]
+    L.sbd_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(Result), ip]
+    L.sbd_server_now.restype = C.c_double
+    L.sbd_server_now.argtypes = [vp]
+    L.sbd_server_stats.argtypes = [vp, up]
+    L.sbd_server_destroy.argtypes = [vp]
     _LIB = L
     return L
 
@@ -342,3 +369,64 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
     finally:
         m.close()
     return E, W
+
+
+class Server:
+    """Continuous-batching solver service on one GPU (sbd_server_* in sbd_tpb.h).
+
+    ``submit`` copies a request's integrals and strings and returns its id at
+    once; ``wait`` blocks until that solve is done and returns its result
+    (energy, residual, converged, steps, status, and the submit / admit / done
+    times on the server clock).  A solve's result is bit-identical alone, in
+    any mix, and to ``MultTPB.diag`` with the same options."""
+
+    def __init__(self, mem_budget_bytes: int, max_active: int = 32):
+        st = C.c_int(0)
+        h = lib().sbd_server_create(int(mem_budget_bytes), int(max_active), C.byref(st))
+        if not h:
+            _check(st.value if st.value else 1)
+        self._h = C.c_void_p(h)
+
+    def submit(self, norb, a, b, *, I0=0.0, h=None, eri=None, sbd: "SBD | None" = None) -> int:
+        sbd = sbd or SBD()
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        eri = np.ascontiguousarray(eri, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        rq = Request(int(norb), float(I0), _ptr(h, C.c_double), _ptr(eri, C.c_double),
+                     _ptr(a, C.c_uint64), len(a), _ptr(b, C.c_uint64), len(b), sbd.options())
+        rid = C.c_uint64(0)
+        _check(lib().sbd_server_submit(self._h, C.byref(rq), C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int) -> dict:
+        r = Result()
+        _check(lib().sbd_server_wait(self._h, C.c_uint64(rid), C.byref(r)))
+        d = r.as_dict()
+        if d["status"] != 0:
+            raise SbdError(f"solve {rid} failed: " + lib().sbd_status_string(d["status"]).decode())
+        return d
+
+    def poll(self, rid: int):
+        r, done = Result(), C.c_int(0)
+        _check(lib().sbd_server_poll(self._h, C.c_uint64(rid), C.byref(r), C.byref(done)))
+        return r.as_dict() if done.value else None
+
+    def now(self) -> float:
+        return lib().sbd_server_now(self._h)
+
+    def stats(self) -> dict:
+        out = np.zeros(2, dtype=np.uint64)
+        _check(lib().sbd_server_stats(self._h, _ptr(out, C.c_uint64)))
+        return {"rounds": int(out[0]), "peak_active": int(out[1])}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sbd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: module globals may be gone
+            pass

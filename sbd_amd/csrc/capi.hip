@@ -18,18 +18,10 @@
 #include <vector>
 
 #include "sbd_internal.hpp"
+#include "davidson_job.hpp"
 
 namespace sbdx {
 int sigma_set_lds_limit(sbd_ctx* ctx);
-}
-
-using namespace sbdx;
-
-namespace {
-
-double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 // Symmetric eigenproblem of order n (column-major, leading dimension lda):
 // cyclic Jacobi rotations, then eigenvalues sorted ascending with their
@@ -80,6 +72,16 @@ void syev_jacobi(int n, double* A, int lda, double* w) {
   for (int i = 0; i < n; i++) w[i] = A[idx[i] + (size_t)idx[i] * lda];
   for (int j = 0; j < n; j++)
     for (int i = 0; i < n; i++) A[i + (size_t)j * lda] = V[i + (size_t)idx[j] * n];
+}
+
+}  // namespace sbdx
+
+using namespace sbdx;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 int set_err(sbd_ctx* ctx, int code, const std::string& msg) {
@@ -215,11 +217,9 @@ struct DevBuf {
   }
 };
 
-// Single-root Davidson on device vectors (davidson.h:707-970), with the
-// vector passes fused as described in vecops.hip.  The new direction is the
-// preconditioned residual, orthogonalised against the subspace by classical
-// Gram-Schmidt with a second (re-orthogonalising) pass when more than
-// 99.9% of its norm cancels.  W: device, in/out.
+// Single-root Davidson on device vectors (davidson.h:707-970), driven pass
+// by pass through DavidsonJob (davidson_job.hpp) with the fused vector passes
+// of vecops.hip.  W: device, in/out (normalised Ritz vector on exit).
 template <typename VT>
 int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double eps,
                   double max_time, int verbose, double* energy, std::vector<double>* hist,
@@ -243,160 +243,121 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
     HC[k] = HCb[k].p;
   }
   SBD_HIP_TRY(ctx, hipMemcpyAsync(C[0], W, sizeof(VT) * N, hipMemcpyDeviceToDevice, st));
-  std::vector<double> H((size_t)nb * nb, 0.0), U((size_t)nb * nb), E(nb), u(nb),
-      o(2 * kMaxVecs + 3);
-  std::vector<const VT*> Cc(nb), HCc(nb);
-  int steps = 0, rc = 0;
-  bool stop = false, converged = false;
-  double E0 = 0.0, normR = 0.0, t_sigma = 0.0;
+  DavidsonJob<VT> job;
+  job.init(C, HC, spare.p, dii, N, max_it, eps);
+  int rc = 0;
+  double t_sigma = 0.0, step_t0 = now_s();
   std::vector<double> step_times;
   hipEvent_t ev0, ev1;
   SBD_HIP_TRY(ctx, hipEventCreate(&ev0));
   SBD_HIP_TRY(ctx, hipEventCreate(&ev1));
-  int last_k = 1;
-  // <HC_j, C_{ib+1}> for j <= ib, carried from one step to the next: by the
-  // symmetry of H these are the off-diagonal entries of the next projected
-  // column, obtained from the sums of the Ritz pass through the Gram-Schmidt
-  // coefficients, so no overlap pass follows the sigma.
-  std::vector<double> hnext(nb, 0.0), hcur(nb, 0.0);
-  for (int it = 0; it < max_it && !stop; it++) {
-    for (int ib = 0; ib < nb; ib++) {
-      const double ts = now_s();
-      for (int k = 0; k <= ib; k++) {
-        Cc[k] = C[k];
-        HCc[k] = HC[k];
-      }
-      SBD_HIP_TRY(ctx, hipEventRecord(ev0, st));
-      double hdiag = 0.0;
-      rc = launch_sigma<VT>(ctx, dii, C[ib], HC[ib], 0, st, &hdiag);
-      if (rc) break;
-      SBD_HIP_TRY(ctx, hipEventRecord(ev1, st));
-      float ms = 0.f;
-      SBD_HIP_TRY(ctx, hipEventSynchronize(ev1));
-      SBD_HIP_TRY(ctx, hipEventElapsedTime(&ms, ev0, ev1));
-      t_sigma += 1e-3 * ms;
-      for (int jb = 0; jb < ib; jb++) {
-        H[jb + (size_t)nb * ib] = hnext[jb];
-        H[ib + (size_t)nb * jb] = hnext[jb];
-      }
-      H[ib + (size_t)nb * ib] = hdiag;
-      for (int jb = 0; jb <= ib; jb++)
-        for (int kb = 0; kb <= ib; kb++) U[jb + (size_t)nb * kb] = H[jb + (size_t)nb * kb];
-      syev_jacobi(ib + 1, U.data(), nb, E.data());
-      E0 = E[0];
-      for (int k = 0; k <= ib; k++) u[k] = U[k];
-      last_k = ib + 1;
-      VT* t = (ib < nb - 1) ? C[ib + 1] : spare.p;
-      rc = vec_ritz_precond<VT>(ctx, Cc.data(), HCc.data(), ib + 1, u.data(), E0, dii, t, N,
-                                o.data(), st);
-      if (rc) break;
-      normR = std::sqrt(o[0]);
-      if (hist) {
-        hist->push_back(E0);
-        hist->push_back(normR);
-        hist->push_back(now_s() - t_start);
-        hist->push_back(sizeof(VT) == 4 ? 32.0 : 64.0);
-      }
-      steps++;
-      if (verbose) {
-        printf(" Davidson iteration %d.%d (tol=%g):", it, ib, normR);
-        for (int pp = 0; pp < std::min(ib + 1, 4); pp++) printf(" %.16g", E[pp]);
-        printf("\n");
-        fflush(stdout);
-      }
-      if (normR < eps) {
-        converged = true;
-        stop = true;
+  auto ptrs = [](const std::vector<VT*>& v, int k) {
+    return std::vector<const VT*>(v.begin(), v.begin() + k);
+  };
+  while (!rc && job.next() != DavidsonJob<VT>::kDone) {
+    switch (job.next()) {
+      case DavidsonJob<VT>::kSigma: {
+        step_t0 = now_s();
+        double td = 0.0;
+        SBD_HIP_TRY(ctx, hipEventRecord(ev0, st));
+        rc = launch_sigma<VT>(ctx, dii, job.sigma_in(), job.sigma_out(), 0, st, &td);
+        if (rc) break;
+        SBD_HIP_TRY(ctx, hipEventRecord(ev1, st));
+        SBD_HIP_TRY(ctx, hipEventSynchronize(ev1));
+        float ms = 0.f;
+        SBD_HIP_TRY(ctx, hipEventElapsedTime(&ms, ev0, ev1));
+        t_sigma += 1e-3 * ms;
+        job.sigma_done(td);
         break;
       }
-      if (ib < nb - 1) {
-        const int k1 = ib + 1;
-        const double* oc = o.data() + 3;       // <C_j, t>
-        const double* oh = o.data() + 3 + k1;  // <HC_j, t>
-        // <HC_j, t - sum_i c_i C_i> = oh_j - sum_i c_i H[i][j]
-        auto project = [&](const double* c, const double* h, double* out) {
-          for (int j = 0; j < k1; j++) {
-            double x = h[j];
-            for (int i = 0; i < k1; i++) x -= c[i] * H[i + (size_t)nb * j];
-            out[j] = x;
-          }
-        };
-        const double tt = o[2];
-        double proj = tt;
-        for (int k = 0; k < k1; k++) proj -= oc[k] * oc[k];
-        double n2 = 0.0;
-        if (tt > 0 && proj > 1e-3 * tt) {
-          const double sc = 1.0 / std::sqrt(proj);
-          rc = vec_gs_update<VT>(ctx, Cc.data(), k1, oc, t, sc, N, &n2, st);
-          if (rc) break;
-          project(oc, oh, hnext.data());
-          for (int j = 0; j < k1; j++) hnext[j] *= sc;
-        } else {
-          // the direction is nearly inside the subspace: orthogonalise twice
-          rc = vec_gs_update<VT>(ctx, Cc.data(), k1, oc, t, 1.0, N, &n2, st);
-          if (rc) break;
-          project(oc, oh, hcur.data());
-          std::vector<double> o2(kMaxVecs);
-          rc = vec_multidot<VT>(ctx, Cc.data(), k1, t, N, o2.data(), st);
-          if (rc) break;
-          double p2 = n2;
-          for (int k = 0; k < k1; k++) p2 -= o2[k] * o2[k];
-          if (!(p2 > 0.0)) {
-            // the new direction lies in the subspace: nothing left to add
-            stop = true;
-            break;
-          }
-          const double sc = 1.0 / std::sqrt(p2);
-          rc = vec_gs_update<VT>(ctx, Cc.data(), k1, o2.data(), t, sc, N, &n2, st);
-          if (rc) break;
-          project(o2.data(), hcur.data(), hnext.data());
-          for (int j = 0; j < k1; j++) hnext[j] *= sc;
+      case DavidsonJob<VT>::kRitz: {
+        const int k = job.k();
+        const auto Cc = ptrs(job.C, k), HCc = ptrs(job.HC, k);
+        rc = vec_ritz_precond<VT>(ctx, Cc.data(), HCc.data(), k, job.ritz_u(), job.E0, dii, job.t(),
+                                  N, job.ritz_sums(), st);
+        if (rc) break;
+        const int ib = job.ib, it = job.it;
+        job.ritz_done();
+        if (hist) {
+          hist->push_back(job.E0);
+          hist->push_back(job.normR);
+          hist->push_back(now_s() - t_start);
+          hist->push_back(sizeof(VT) == 4 ? 32.0 : 64.0);
         }
+        if (verbose) {
+          printf(" Davidson iteration %d.%d (tol=%g):", it, ib, job.normR);
+          for (int pp = 0; pp < std::min(ib + 1, 4); pp++) printf(" %.16g", job.Evals[pp]);
+          printf("\n");
+          fflush(stdout);
+        }
+        break;
       }
-      // max_time guard (davidson.h:926-949): stop if the next step would end late
-      step_times.push_back(now_s() - ts);
+      case DavidsonJob<VT>::kGs: {
+        const int k = job.k();
+        const auto Cc = ptrs(job.C, k);
+        double n2 = 0.0;
+        rc = vec_gs_update<VT>(ctx, Cc.data(), k, job.gs_coef(), job.t(), job.gs_scale(), N, &n2,
+                               st);
+        if (!rc) job.gs_done(n2);
+        break;
+      }
+      case DavidsonJob<VT>::kDots: {
+        const int k = job.k();
+        const auto Cc = ptrs(job.C, k);
+        rc = vec_multidot<VT>(ctx, Cc.data(), k, job.t(), N, job.dots_out(), st);
+        if (!rc) job.dots_done();
+        break;
+      }
+      case DavidsonJob<VT>::kLincomb: {
+        const int k = job.lincomb_k();
+        const auto Cc = ptrs(job.C, k);
+        double n2 = 0.0;
+        rc = vec_lincomb<VT>(ctx, Cc.data(), k, job.ritz_u(), 1.0, job.lincomb_out(), N, &n2, st);
+        if (!rc) job.lincomb_done(n2);
+        break;
+      }
+      case DavidsonJob<VT>::kScale:
+        rc = vec_scale<VT>(ctx, job.lincomb_out(), job.scale_factor(), N, st);
+        if (!rc) job.scale_done();
+        break;
+      default:
+        break;
+    }
+    if (rc) break;
+    // max_time guard after each step (davidson.h:926-949): stop if the next
+    // step would end late
+    const auto nx = job.next();
+    if (nx == DavidsonJob<VT>::kSigma || nx == DavidsonJob<VT>::kLincomb) {
+      step_times.push_back(now_s() - step_t0);
+      step_t0 = now_s();
       double avg = 0.0;
       for (double x : step_times) avg += x;
       avg /= step_times.size();
       if ((now_s() - t_start) + avg > max_time) {
         if (verbose) fprintf(stderr, "halting iterations due to max_time limit\n");
-        stop = true;
-        break;
+        job.stop();
       }
-    }
-    if (rc) break;
-    if (!stop) {
-      // restart with C0 = W (davidson.h:958-962); |W| = 1 up to the
-      // orthogonality of the basis, rescaled when it drifts
-      double n2 = 0.0;
-      rc = vec_lincomb<VT>(ctx, Cc.data(), last_k, u.data(), 1.0, spare.p, N, &n2, st);
-      if (rc) break;
-      if (std::fabs(n2 - 1.0) > 1e-12) {
-        rc = vec_scale<VT>(ctx, spare.p, 1.0 / std::sqrt(n2), N, st);
-        if (rc) break;
-      }
-      std::swap(C[0], spare.p);
-      // keep ownership consistent with the buffers
-      for (int k = 0; k < nb; k++)
-        if (Cb[k].p == spare.p) Cb[k].p = C[0];
     }
   }
+  // the buffers may have been rotated by restarts: keep ownership consistent
+  for (int k = 0; k < nb; k++) Cb[k].p = job.C[k];
+  spare.p = job.spare;
   if (!rc) {
     // W = normalised Ritz vector of the last step
     double n2 = 0.0;
-    for (int k = 0; k < last_k; k++) Cc[k] = C[k];
-    rc = vec_lincomb<VT>(ctx, Cc.data(), last_k, u.data(), 1.0, W, N, &n2, st);
+    const auto Cc = ptrs(job.C, job.last_k);
+    rc = vec_lincomb<VT>(ctx, Cc.data(), job.last_k, job.ritz_u(), 1.0, W, N, &n2, st);
     if (!rc) rc = vec_scale<VT>(ctx, W, 1.0 / std::sqrt(n2), N, st);
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = SBD_ERR_DEVICE;
   }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
-  if (energy) *energy = E0;
+  if (energy) *energy = job.E0;
   if (rep) {
-    rep->steps += steps;
-    rep->converged = converged ? 1 : 0;
-    rep->residual = normR;
-    rep->energy = E0;
+    rep->steps += job.steps;
+    rep->converged = job.converged ? 1 : 0;
+    rep->residual = job.normR;
+    rep->energy = job.E0;
     rep->t_sigma += t_sigma;
   }
   return rc;

@@ -233,4 +233,60 @@ template <typename VT>
 int vec_scale(sbd_ctx* ctx, VT* x, double s, uint64_t n, hipStream_t st);
 template <typename VS, typename VD>
 int vec_convert(sbd_ctx* ctx, const VS* x, VD* y, uint64_t n, hipStream_t st);
+
+// Scratch of a batching engine (device jobs, partials and results; pinned
+// host results), grown on demand.
+struct BatchScratch {
+  void* d_jobs = nullptr;
+  size_t jobs_cap = 0;
+  double* d_part = nullptr;
+  size_t part_cap = 0;
+  double* d_out = nullptr;
+  size_t out_cap = 0;
+  double* h_out = nullptr;
+  size_t hout_cap = 0;
+  void* d_fin = nullptr;
+  size_t fin_cap = 0;
+};
+int batch_reserve(BatchScratch& sc, size_t job_bytes, size_t parts, size_t outs);
+void batch_free(BatchScratch& sc);
+
+// One job of a batched Ritz pass (arguments of vec_ritz_precond).
+template <typename VT>
+struct RitzArgs {
+  const VT* const* C;
+  const VT* const* HC;
+  int k;
+  const double* u;
+  double E0;
+  const VT* dii;
+  VT* t;
+  uint64_t n;
+};
+// out[j]: 3 + 2 k_j sums in the layout of vec_ritz_precond
+template <typename VT>
+int vec_ritz_batch(BatchScratch& sc, const RitzArgs<VT>* a, int njobs, double* const* out,
+                   hipStream_t st);
+// One job of a batched Gram-Schmidt pass (arguments of vec_gs_update).
+template <typename VT>
+struct GsArgs {
+  const VT* const* C;
+  int k;
+  const double* o;
+  VT* t;
+  double scale;
+  uint64_t n;
+};
+template <typename VT>
+int vec_gs_batch(BatchScratch& sc, const GsArgs<VT>* a, int njobs, double* out_norm2,
+                 hipStream_t st);
+// One job of a batched sigma: W = H T (accumulate 0) with <T, W> returned.
+struct SigmaItem {
+  sbd_ctx* ctx;
+  const double* hii;
+  const double* T;
+  double* W;
+};
+int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* tw_out,
+                       hipStream_t st);
 }  // namespace sbdx

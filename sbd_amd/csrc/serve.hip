@@ -1,0 +1,494 @@
+// Serving engine: many independent TPB solves on one GPU with continuous
+// batching.
+//
+// Requests arrive at any time (sbd_server_submit).  An admission thread takes
+// them in arrival order, reserves their device footprint against the budget,
+// builds their context (tables, diagonal, start vector, Davidson buffers) and
+// hands them to the engine.  The engine thread advances every admitted solve
+// one Davidson pass at a time (DavidsonJob, davidson_job.hpp) and runs each
+// kind of pass for all solves that need it in one launch: one sigma launch
+// (k_sigma_batch), one Ritz launch, one Gram-Schmidt launch per round.  A solve
+// that finishes frees its memory at once, and requests that arrived meanwhile
+// are admitted as memory and slots allow, so their work joins the next round.
+//
+// Each solve takes the same number of workgroups in a batch as it would
+// alone and sums its own partials in a fixed order, so its iterates -- and its
+// energy -- are bit-identical alone, in any mix, and to sbd_diag with the same
+// options (FP64; init 0 or 1).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "davidson_job.hpp"
+#include "sbd_internal.hpp"
+
+using namespace sbdx;
+
+namespace {
+
+double clock_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  uint64_t id = 0;
+  int norb = 0;
+  double I0 = 0.0;
+  std::vector<double> h1, eri;
+  std::vector<uint64_t> a, b;
+  sbd_options opt{};
+  double t_submit = 0.0;
+  uint64_t bytes = 0;  // reserved device footprint
+};
+
+// One admitted solve with its device buffers.
+struct Solve {
+  Request req;
+  sbd_ctx* ctx = nullptr;
+  std::vector<double*> C, HC;
+  double* spare = nullptr;
+  double* hii = nullptr;
+  DavidsonJob<double> job;
+  double t_admit = 0.0;
+  int status = SBD_OK;
+
+  void release() {
+    for (double* p : C)
+      if (p) (void)hipFree(p);
+    for (double* p : HC)
+      if (p) (void)hipFree(p);
+    if (spare) (void)hipFree(spare);
+    if (hii) (void)hipFree(hii);
+    C.clear();
+    HC.clear();
+    spare = hii = nullptr;
+    if (ctx) sbd_ctx_destroy(ctx);
+    ctx = nullptr;
+  }
+};
+
+// Device bytes of one solve: 2 max_nb + 2 vectors (C, HC, spare, hii) plus
+// tables and integrals (bounded as in sbd_diag_footprint).
+uint64_t footprint(const Request& r) {
+  const uint64_t N = (uint64_t)r.a.size() * r.b.size();
+  const uint64_t nb = (uint64_t)r.opt.max_nb;
+  const uint64_t L = (uint64_t)r.norb;
+  uint64_t bytes = (2 * nb + 2) * N * 8;
+  bytes += 8 * (4 * L * L + r.eri.size() + 2 * L * L + 4 * L * L * L);
+  auto recs = [&](const std::vector<uint64_t>& s) {
+    const uint64_t n = s.empty() ? 0 : (uint64_t)__builtin_popcountll(s[0]);
+    const uint64_t v1 = n * (L - n), v2 = n * (n - 1) / 2 * (L - n) * (L - n - 1) / 2;
+    return s.size() * std::min<uint64_t>(v1 + v2, s.size());
+  };
+  // records, ELL copies (at most 2x padding, 36 bytes a slot) and caches
+  bytes += 64 * (recs(r.a) + recs(r.b)) + 16 * (r.a.size() + r.b.size() + 2);
+  const uint64_t singles = std::min<uint64_t>(
+      (uint64_t)(r.a.empty() ? 0 : __builtin_popcountll(r.a[0])) * L, r.a.size());
+  bytes += 8 * singles * (r.a.size() + r.b.size()) * 2;
+  return bytes;
+}
+
+}  // namespace
+
+struct sbd_server {
+  int device = 0;
+  uint64_t budget = 0, used = 0;
+  int max_active = 0;
+  std::mutex mu;
+  std::condition_variable cv_admit, cv_engine, cv_done;
+  std::deque<Request> pending;
+  std::deque<Solve*> ready;
+  int preparing = 0, running = 0;
+  std::map<uint64_t, sbd_result> results;
+  std::map<uint64_t, int> state;  // 0 queued, 1 admitted, 2 done
+  bool stopping = false;
+  uint64_t next_id = 1;
+  std::thread admitter, engine;
+  hipStream_t st = nullptr;
+  BatchScratch sc;
+  double t0 = 0.0;
+  uint64_t rounds = 0, peak_active = 0;
+  std::string last_error;
+
+  double now() const { return clock_s() - t0; }
+
+  void finish(Solve* s, double energy) {
+    sbd_result r{};
+    r.energy = energy;
+    r.converged = s->job.converged ? 1 : 0;
+    r.steps = s->job.steps;
+    r.residual = s->job.normR;
+    r.t_submit = s->req.t_submit;
+    r.t_admit = s->t_admit;
+    r.t_done = now();
+    r.status = s->status;
+    s->release();
+    std::lock_guard<std::mutex> g(mu);
+    used -= s->req.bytes;
+    results[s->req.id] = r;
+    state[s->req.id] = 2;
+    running--;
+    cv_done.notify_all();
+    cv_admit.notify_all();
+    delete s;
+  }
+
+  // ---- admission: context, tables, diagonal, start vector, buffers
+  int prepare(Solve* s) {
+    const Request& q = s->req;
+    int st0 = 0;
+    s->ctx = sbd_ctx_create(q.norb, q.I0, q.h1.data(), q.eri.data(), q.a.data(), q.a.size(),
+                            q.b.data(), q.b.size(), &st0);
+    if (!s->ctx) return st0 ? st0 : SBD_ERR_DEVICE;
+    int rc = sbd_make_helpers(s->ctx);
+    if (rc) return rc;
+    const uint64_t N = s->ctx->N;
+    const int nb = q.opt.max_nb;
+    s->C.assign(nb, nullptr);
+    s->HC.assign(nb, nullptr);
+    for (int k = 0; k < nb; k++) {
+      if (hipMalloc(&s->C[k], 8 * N) != hipSuccess || hipMalloc(&s->HC[k], 8 * N) != hipSuccess)
+        return SBD_ERR_MEMORY;
+    }
+    if (hipMalloc(&s->spare, 8 * N) != hipSuccess || hipMalloc(&s->hii, 8 * N) != hipSuccess)
+      return SBD_ERR_MEMORY;
+    rc = launch_hdiag(s->ctx, s->hii, nullptr);
+    if (rc) return rc;
+    // start vector as sbd_diag makes it (BasisInitVector, davidson.h:144-184)
+    std::vector<double> W0(N, 0.0);
+    if (q.opt.init == 0) {
+      W0[0] = 1.0;
+    } else {
+      uint64_t sd = q.opt.seed;
+      double nrm = 0.0;
+      for (uint64_t i = 0; i < N; i++) {
+        sd += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = sd;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        W0[i] = 2.0 * ((z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+        nrm += W0[i] * W0[i];
+      }
+      nrm = 1.0 / std::sqrt(nrm);
+      for (auto& x : W0) x *= nrm;
+    }
+    if (hipMemcpy(s->C[0], W0.data(), 8 * N, hipMemcpyHostToDevice) != hipSuccess)
+      return SBD_ERR_DEVICE;
+    if (hipDeviceSynchronize() != hipSuccess) return SBD_ERR_DEVICE;
+    s->job.init(s->C, s->HC, s->spare, s->hii, N, q.opt.max_it, q.opt.eps);
+    return SBD_OK;
+  }
+
+  void admit_loop() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      Request q;
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv_admit.wait(g, [&] {
+          if (stopping && pending.empty()) return true;
+          if (pending.empty()) return false;
+          const uint64_t need = pending.front().bytes;
+          const bool slot = running + preparing + (int)ready.size() < max_active;
+          const bool mem = used + need <= budget || (running + preparing + ready.size() == 0);
+          return slot && mem;
+        });
+        if (pending.empty()) return;  // stopping
+        q = std::move(pending.front());
+        pending.pop_front();
+        used += q.bytes;
+        preparing++;
+        state[q.id] = 1;
+      }
+      Solve* s = new Solve();
+      s->req = std::move(q);
+      const int rc = prepare(s);
+      std::unique_lock<std::mutex> g(mu);
+      preparing--;
+      if (rc) {
+        s->status = rc;
+        last_error = s->ctx ? s->ctx->last_error : "request preparation failed";
+        running++;  // finish() counts it out
+        g.unlock();
+        finish(s, 0.0);
+        continue;
+      }
+      ready.push_back(s);
+      cv_engine.notify_all();
+    }
+  }
+
+  // ---- engine: one round advances every active solve through its passes
+  void engine_loop() {
+    (void)hipSetDevice(device);
+    std::vector<Solve*> active;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv_engine.wait(g, [&] {
+          return !ready.empty() || !active.empty() ||
+                 (stopping && pending.empty() && preparing == 0);
+        });
+        while (!ready.empty()) {
+          Solve* s = ready.front();
+          ready.pop_front();
+          s->t_admit = now();
+          running++;
+          active.push_back(s);
+        }
+        if (active.empty() && stopping && pending.empty() && preparing == 0) return;
+      }
+      peak_active = std::max<uint64_t>(peak_active, active.size());
+      rounds++;
+      round(active);
+      // completed solves leave the batch
+      std::vector<Solve*> keep;
+      for (Solve* s : active) {
+        if (s->job.next() == DavidsonJob<double>::kDone || s->status) complete(s);
+        else keep.push_back(s);
+      }
+      active.swap(keep);
+    }
+  }
+
+  void fail(Solve* s, int rc) {
+    s->status = rc;
+    s->job.stop();
+  }
+
+  void round(std::vector<Solve*>& active) {
+    using DJ = DavidsonJob<double>;
+    // sigma for every solve that needs one, in one launch
+    {
+      std::vector<SigmaItem> items;
+      std::vector<Solve*> who;
+      for (Solve* s : active)
+        if (s->job.next() == DJ::kSigma) {
+          items.push_back(SigmaItem{s->ctx, s->hii, s->job.sigma_in(), s->job.sigma_out()});
+          who.push_back(s);
+        }
+      if (!items.empty()) {
+        std::vector<double> tw(items.size(), 0.0);
+        const int rc = launch_sigma_batch(sc, items.data(), (int)items.size(), tw.data(), st);
+        for (size_t q = 0; q < who.size(); q++) {
+          if (rc) fail(who[q], rc);
+          else who[q]->job.sigma_done(tw[q]);
+        }
+      }
+    }
+    // Ritz pass
+    {
+      std::vector<RitzArgs<double>> args;
+      std::vector<std::vector<const double*>> cp, hp;
+      std::vector<double*> outs;
+      std::vector<Solve*> who;
+      for (Solve* s : active)
+        if (s->job.next() == DJ::kRitz) who.push_back(s);
+      cp.resize(who.size());
+      hp.resize(who.size());
+      for (size_t q = 0; q < who.size(); q++) {
+        Solve* s = who[q];
+        const int k = s->job.k();
+        cp[q].assign(s->job.C.begin(), s->job.C.begin() + k);
+        hp[q].assign(s->job.HC.begin(), s->job.HC.begin() + k);
+        args.push_back(RitzArgs<double>{cp[q].data(), hp[q].data(), k, s->job.ritz_u(), s->job.E0,
+                                        s->hii, s->job.t(), s->ctx->N});
+        outs.push_back(s->job.ritz_sums());
+      }
+      if (!who.empty()) {
+        const int rc = vec_ritz_batch<double>(sc, args.data(), (int)args.size(), outs.data(), st);
+        for (Solve* s : who) {
+          if (rc) fail(s, rc);
+          else s->job.ritz_done();
+        }
+      }
+    }
+    // Gram-Schmidt
+    {
+      std::vector<GsArgs<double>> args;
+      std::vector<std::vector<const double*>> cp;
+      std::vector<Solve*> who;
+      for (Solve* s : active)
+        if (s->job.next() == DJ::kGs) who.push_back(s);
+      cp.resize(who.size());
+      for (size_t q = 0; q < who.size(); q++) {
+        Solve* s = who[q];
+        const int k = s->job.k();
+        cp[q].assign(s->job.C.begin(), s->job.C.begin() + k);
+        args.push_back(GsArgs<double>{cp[q].data(), k, s->job.gs_coef(), s->job.t(),
+                                      s->job.gs_scale(), s->ctx->N});
+      }
+      if (!who.empty()) {
+        std::vector<double> n2(who.size(), 0.0);
+        const int rc = vec_gs_batch<double>(sc, args.data(), (int)args.size(), n2.data(), st);
+        for (size_t q = 0; q < who.size(); q++) {
+          if (rc) fail(who[q], rc);
+          else who[q]->job.gs_done(n2[q]);
+        }
+      }
+    }
+    // the rare passes (re-orthogonalisation, restart) one solve at a time,
+    // until each solve waits for its next sigma or is done
+    for (Solve* s : active) {
+      for (;;) {
+        const auto op = s->job.next();
+        if (op == DJ::kSigma || op == DJ::kDone || op == DJ::kRitz || op == DJ::kGs) break;
+        int rc = SBD_OK;
+        const uint64_t N = s->ctx->N;
+        if (op == DJ::kDots) {
+          const int k = s->job.k();
+          std::vector<const double*> cp(s->job.C.begin(), s->job.C.begin() + k);
+          rc = vec_multidot<double>(s->ctx, cp.data(), k, s->job.t(), N, s->job.dots_out(), st);
+          if (!rc) s->job.dots_done();
+        } else if (op == DJ::kLincomb) {
+          const int k = s->job.lincomb_k();
+          std::vector<const double*> cp(s->job.C.begin(), s->job.C.begin() + k);
+          double n2 = 0.0;
+          rc = vec_lincomb<double>(s->ctx, cp.data(), k, s->job.ritz_u(), 1.0,
+                                   s->job.lincomb_out(), N, &n2, st);
+          if (!rc) s->job.lincomb_done(n2);
+        } else if (op == DJ::kScale) {
+          rc = vec_scale<double>(s->ctx, s->job.lincomb_out(), s->job.scale_factor(), N, st);
+          if (!rc) s->job.scale_done();
+        }
+        if (rc) {
+          fail(s, rc);
+          break;
+        }
+      }
+    }
+  }
+
+  // final vector and E = <W, H W> with a fresh sigma, as sbd_diag does
+  void complete(Solve* s) {
+    double E = 0.0;
+    if (!s->status) {
+      const uint64_t N = s->ctx->N;
+      auto& j = s->job;
+      std::vector<const double*> cp(j.C.begin(), j.C.begin() + j.last_k);
+      double n2 = 0.0;
+      double* W = j.spare;
+      int rc = vec_lincomb<double>(s->ctx, cp.data(), j.last_k, j.ritz_u(), 1.0, W, N, &n2, st);
+      if (!rc) rc = vec_scale<double>(s->ctx, W, 1.0 / std::sqrt(n2), N, st);
+      double* HW = j.HC[0];
+      if (!rc) rc = launch_sigma<double>(s->ctx, s->hii, W, HW, 0, st);
+      const double* X[1] = {W};
+      if (!rc) rc = vec_multidot<double>(s->ctx, X, 1, HW, N, &E, st);
+      if (rc) s->status = rc;
+    }
+    // buffers may have been rotated by restarts
+    s->C = s->job.C;
+    s->spare = s->job.spare;
+    finish(s, E);
+  }
+};
+
+extern "C" {
+
+sbd_server* sbd_server_create(uint64_t mem_budget, int max_active, int* status) {
+  if (max_active < 1 || mem_budget == 0) {
+    if (status) *status = SBD_ERR_ARG;
+    return nullptr;
+  }
+  sbd_server* sv = new sbd_server();
+  sv->budget = mem_budget;
+  sv->max_active = max_active;
+  sv->t0 = clock_s();
+  if (hipGetDevice(&sv->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess) {
+    if (status) *status = SBD_ERR_DEVICE;
+    delete sv;
+    return nullptr;
+  }
+  sv->admitter = std::thread([sv] { sv->admit_loop(); });
+  sv->engine = std::thread([sv] { sv->engine_loop(); });
+  if (status) *status = SBD_OK;
+  return sv;
+}
+
+int sbd_server_submit(sbd_server* sv, const sbd_request* rq, uint64_t* id) {
+  if (!sv || !rq || !rq->h1 || !rq->eri_packed || !rq->a_strings || !rq->b_strings ||
+      rq->norb < 1 || rq->norb > 64 || rq->na == 0 || rq->nb == 0)
+    return SBD_ERR_ARG;
+  if (rq->opt.method != 0 || rq->opt.precision != 0 || rq->opt.max_nb < 1 ||
+      rq->opt.max_nb > kMaxVecs || rq->opt.max_it < 1 || (rq->opt.init != 0 && rq->opt.init != 1))
+    return SBD_ERR_ARG;
+  Request q;
+  q.norb = rq->norb;
+  q.I0 = rq->I0;
+  const size_t L = (size_t)rq->norb, B = L * (L + 1) / 2;
+  q.h1.assign(rq->h1, rq->h1 + L * L);
+  q.eri.assign(rq->eri_packed, rq->eri_packed + B * (B + 1) / 2);
+  q.a.assign(rq->a_strings, rq->a_strings + rq->na);
+  q.b.assign(rq->b_strings, rq->b_strings + rq->nb);
+  q.opt = rq->opt;
+  q.bytes = footprint(q);
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (sv->stopping) return SBD_ERR_STATE;
+  q.id = sv->next_id++;
+  q.t_submit = sv->now();
+  if (id) *id = q.id;
+  sv->state[q.id] = 0;
+  sv->pending.push_back(std::move(q));
+  sv->cv_admit.notify_all();
+  return SBD_OK;
+}
+
+int sbd_server_wait(sbd_server* sv, uint64_t id, sbd_result* out) {
+  if (!sv) return SBD_ERR_ARG;
+  std::unique_lock<std::mutex> g(sv->mu);
+  if (!sv->state.count(id)) return SBD_ERR_ARG;
+  sv->cv_done.wait(g, [&] { return sv->state[id] == 2; });
+  if (out) *out = sv->results[id];
+  return SBD_OK;
+}
+
+int sbd_server_poll(sbd_server* sv, uint64_t id, sbd_result* out, int* done) {
+  if (!sv || !done) return SBD_ERR_ARG;
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (!sv->state.count(id)) return SBD_ERR_ARG;
+  *done = sv->state[id] == 2;
+  if (*done && out) *out = sv->results[id];
+  return SBD_OK;
+}
+
+double sbd_server_now(const sbd_server* sv) { return sv ? sv->now() : 0.0; }
+
+int sbd_server_stats(const sbd_server* sv, uint64_t* out) {
+  if (!sv || !out) return SBD_ERR_ARG;
+  out[0] = sv->rounds;
+  out[1] = sv->peak_active;
+  return SBD_OK;
+}
+
+void sbd_server_destroy(sbd_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->stopping = true;
+    sv->cv_admit.notify_all();
+    sv->cv_engine.notify_all();
+  }
+  if (sv->admitter.joinable()) sv->admitter.join();
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->cv_engine.notify_all();
+  }
+  if (sv->engine.joinable()) sv->engine.join();
+  batch_free(sv->sc);
+  if (sv->st) (void)hipStreamDestroy(sv->st);
+  delete sv;
+}
+
+}  // extern "C"

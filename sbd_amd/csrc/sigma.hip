@@ -13,7 +13,9 @@
 // excitations of (ia, ib) land in that row.  Alpha-pair records are
 // workgroup-uniform (scalar loads); beta-pair records are per column.  There
 // are no atomics: each output element has one writer.
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "sbd_internal.hpp"
 
@@ -122,6 +124,14 @@ struct SigmaArgs {
   uint32_t Sb;       // number of beta single records
   const double* G;   // [2][L][L][2L]
   const double* I2;  // packed ERI
+};
+
+template <typename VT>
+struct SigmaJob {
+  SigmaArgs<VT> p;
+  const EllF32* ell_f32;
+  uint32_t blk0, ngrp;  // first workgroup and number of row groups
+  double* dot_part;     // ngrp * 16 partials of <T, W>
 };
 
 // OneExcite energy (before the sign): val + sum over occupied spin orbitals j
@@ -283,34 +293,22 @@ struct EllRead<float> {
 // rows fit LDS together) each beta-double slot is fetched once for all rows.
 // Every pointer is a restrict-qualified kernel argument: W aliases no input,
 // so the workgroup-uniform alpha records are fetched with scalar loads.
+// Row groups bid, bid + nblk, ... of one sigma (the workgroup's share);
+// dot_part (optional) receives this workgroup's per-wavefront <T, W> sums at
+// [bid * 16 + wave].
 template <typename VT, typename AT, bool kRowLds, bool kCache, int kRows>
-__global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
-    const uint64_t* __restrict__ a_, const uint64_t* __restrict__ b_, uint64_t na_, uint64_t nb_,
-    int L_, const VT* __restrict__ hii_, const VT* __restrict__ T_, VT* __restrict__ W_,
-    int accumulate_, const uint32_t* __restrict__ as_off_, const PairRec* __restrict__ as_rec_,
-    const uint32_t* __restrict__ ad_off_, const PairRec* __restrict__ ad_rec_,
-    const uint32_t* __restrict__ bs_off_, const PairRec* __restrict__ bs_rec_,
-    const uint32_t* __restrict__ bd_off_, const uint32_t* __restrict__ bd_ell_ket_,
-    const ValMask* __restrict__ bd_ell_vm_, const EllF32* __restrict__ bd_ell_f32_,
-    const uint32_t* __restrict__ bd_ell_off_, const uint32_t* __restrict__ bd_ell_col_,
-    const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
-    const double* __restrict__ G_, const double* __restrict__ I2_,
-    double* __restrict__ dot_part) {
+__device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
+                                       const EllF32* __restrict__ bd_ell_f32_, uint32_t bid,
+                                       uint32_t ngrp, double* __restrict__ dot_part) {
   static_assert(std::is_same<VT, AT>::value, "partial sums round-trip through W");
   static_assert(kRowLds || kRows == 1, "row groups need the rows in LDS");
-  const SigmaArgs<VT> p{a_,      b_,          na_,         nb_,         L_,          hii_,
-                        T_,      W_,          accumulate_, as_off_,     as_rec_,     ad_off_,
-                        ad_rec_, bs_off_,     bs_rec_,     bd_off_,     bd_ell_ket_, bd_ell_vm_,
-                        bd_ell_off_, bd_ell_col_, Ea_,     Eb_,         Sb_,         G_,
-                        I2_};
   extern __shared__ unsigned char lds_raw[];
   VT* trows = reinterpret_cast<VT*>(lds_raw);
   const uint32_t nb = (uint32_t)p.nb;
   const int L = p.L;
   const uint32_t nblk = (nb + kSigBlock - 1) / kSigBlock;
   double tw = 0.0;  // sum of T * W over this thread's outputs (when dot_part is set)
-  for (uint64_t ia0 = (uint64_t)blockIdx.x * kRows; ia0 < p.na;
-       ia0 += (uint64_t)gridDim.x * kRows) {
+  for (uint64_t ia0 = (uint64_t)bid * kRows; ia0 < p.na; ia0 += (uint64_t)ngrp * kRows) {
     const int nr = (int)(p.na - ia0 < (uint64_t)kRows ? p.na - ia0 : kRows);  // uniform
     __syncthreads();
     if (kRowLds) {
@@ -461,8 +459,47 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
   }
   if (dot_part) {
     for (int o = 32; o > 0; o >>= 1) tw += __shfl_down(tw, o, 64);
-    if ((threadIdx.x & 63) == 0) dot_part[blockIdx.x * (kSigBlock / 64) + threadIdx.x / 64] = tw;
+    if ((threadIdx.x & 63) == 0) dot_part[bid * (kSigBlock / 64) + threadIdx.x / 64] = tw;
   }
+}
+
+template <typename VT, typename AT, bool kRowLds, bool kCache, int kRows>
+__global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
+    const uint64_t* __restrict__ a_, const uint64_t* __restrict__ b_, uint64_t na_, uint64_t nb_,
+    int L_, const VT* __restrict__ hii_, const VT* __restrict__ T_, VT* __restrict__ W_,
+    int accumulate_, const uint32_t* __restrict__ as_off_, const PairRec* __restrict__ as_rec_,
+    const uint32_t* __restrict__ ad_off_, const PairRec* __restrict__ ad_rec_,
+    const uint32_t* __restrict__ bs_off_, const PairRec* __restrict__ bs_rec_,
+    const uint32_t* __restrict__ bd_off_, const uint32_t* __restrict__ bd_ell_ket_,
+    const ValMask* __restrict__ bd_ell_vm_, const EllF32* __restrict__ bd_ell_f32_,
+    const uint32_t* __restrict__ bd_ell_off_, const uint32_t* __restrict__ bd_ell_col_,
+    const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
+    const double* __restrict__ G_, const double* __restrict__ I2_,
+    double* __restrict__ dot_part) {
+  const SigmaArgs<VT> p{a_,      b_,          na_,         nb_,         L_,          hii_,
+                        T_,      W_,          accumulate_, as_off_,     as_rec_,     ad_off_,
+                        ad_rec_, bs_off_,     bs_rec_,     bd_off_,     bd_ell_ket_, bd_ell_vm_,
+                        bd_ell_off_, bd_ell_col_, Ea_,     Eb_,         Sb_,         G_,
+                        I2_};
+  sigma_rows_body<VT, AT, kRowLds, kCache, kRows>(p, bd_ell_f32_, blockIdx.x, gridDim.x,
+                                                  dot_part);
+}
+
+// Several sigmas in one launch (the serving engine's batch): job j owns
+// workgroups [blk0, blk0 + ngrp) and each of them one row group, so a job's
+// results, including its <T, W> partials, do not depend on the other jobs.
+template <typename VT, int kRows>
+__global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* __restrict__ jobs,
+                                                           int njobs) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const SigmaJob<VT>& J = jobs[lo];
+  sigma_rows_body<VT, VT, true, true, kRows>(J.p, J.ell_f32, blockIdx.x - J.blk0, J.ngrp,
+                                             J.dot_part);
 }
 
 // Sum of n partials in a fixed order: thread t adds partials t, t + 1024, ...
@@ -511,6 +548,123 @@ int build_single_cache(sbd_ctx* ctx, hipStream_t st) {
   return SBD_OK;
 }
 
+struct SumJob {
+  const double* part;
+  uint64_t n;
+  double* out;
+};
+// k_sum_fixed for several jobs, one block each (same arithmetic)
+__global__ __launch_bounds__(kSumBlock) void k_sum_fixed_batch(const SumJob* __restrict__ jobs) {
+  __shared__ double red[kSumBlock];
+  const SumJob& J = jobs[blockIdx.x];
+  double s = 0.0, c = 0.0;
+  for (uint64_t i = threadIdx.x; i < J.n; i += kSumBlock) {
+    const double y = J.part[i] - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kSumBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J.out[0] = red[0];
+}
+
+static SigmaArgs<double> sigma_args(const sbd_ctx* ctx, const double* hii, const double* T,
+                                    double* W) {
+  const SpinTables& A = ctx->ta;
+  const SpinTables& B = ctx->tb;
+  return SigmaArgs<double>{ctx->d_a,  ctx->d_b,   ctx->na,    ctx->nb,   ctx->norb, hii,
+                           T,         W,          0,          A.s_off,   A.s_rec,   A.d_off,
+                           A.d_rec,   B.s_off,    B.s_rec,    B.d_off,   B.ell_ket, B.ell_vm,
+                           B.ell_off, B.ell_col,  ctx->d_Ea,  ctx->d_Eb, (uint32_t)B.n_s,
+                           ctx->d_G,  ctx->d_eri};
+}
+
+// rows per workgroup for one context (a property of the context alone, so a
+// job computes the same way in any batch)
+static int sigma_rows_for(const sbd_ctx* ctx, size_t elem) {
+  const size_t row_bytes = elem * ctx->nb;
+  return (2 * row_bytes <= (size_t)ctx->lds_max && ctx->na >= 2) ? 2 : 1;
+}
+
+int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* tw_out,
+                       hipStream_t st) {
+  if (n < 1) return SBD_OK;
+  std::vector<int> direct, rows1, rows2;
+  for (int j = 0; j < n; j++) {
+    const sbd_ctx* c = items[j].ctx;
+    if (!c->tables_built) return SBD_ERR_STATE;
+    const bool cache = (c->ta.n_s == 0 || c->d_Ea) && (c->tb.n_s == 0 || c->d_Eb);
+    const bool lds = sizeof(double) * c->nb <= (size_t)c->lds_max;
+    const uint64_t groups = (c->na + 1) / 2;
+    if (!cache || !lds || groups > kSigMaxGrid) direct.push_back(j);
+    else if (sigma_rows_for(c, sizeof(double)) == 2) rows2.push_back(j);
+    else rows1.push_back(j);
+  }
+  // jobs outside the batched form: their own launch (same row groups and sums)
+  for (int j : direct) {
+    int rc = launch_sigma<double>(items[j].ctx, items[j].hii, items[j].T, items[j].W, 0, st,
+                                  &tw_out[j]);
+    if (rc) return rc;
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    const std::vector<int>& sel = pass ? rows2 : rows1;
+    if (sel.empty()) continue;
+    const int R = pass ? 2 : 1;
+    std::vector<SigmaJob<double>> jobs(sel.size());
+    std::vector<SumJob> sums(sel.size());
+    size_t parts = 0, shm = 0;
+    uint32_t blk = 0;
+    for (size_t q = 0; q < sel.size(); q++) {
+      const SigmaItem& it = items[sel[q]];
+      SigmaJob<double>& J = jobs[q];
+      J.p = sigma_args(it.ctx, it.hii, it.T, it.W);
+      J.ell_f32 = it.ctx->tb.ell_f32;
+      J.ngrp = (uint32_t)((it.ctx->na + R - 1) / R);
+      J.blk0 = blk;
+      blk += J.ngrp;
+      parts += (size_t)J.ngrp * (kSigBlock / 64);
+      shm = std::max(shm, (size_t)R * sizeof(double) * it.ctx->nb);
+    }
+    int rc = batch_reserve(sc, std::max(sizeof(SigmaJob<double>), sizeof(SumJob)) * sel.size(),
+                           parts, sel.size());
+    if (rc) return rc;
+    size_t off = 0;
+    for (size_t q = 0; q < sel.size(); q++) {
+      jobs[q].dot_part = sc.d_part + off;
+      sums[q] = SumJob{sc.d_part + off, (uint64_t)jobs[q].ngrp * (kSigBlock / 64), sc.d_out + q};
+      off += (size_t)jobs[q].ngrp * (kSigBlock / 64);
+    }
+    if (hipMemcpyAsync(sc.d_jobs, jobs.data(), sizeof(SigmaJob<double>) * jobs.size(),
+                       hipMemcpyHostToDevice, st) != hipSuccess)
+      return SBD_ERR_DEVICE;
+    const SigmaJob<double>* dj = (const SigmaJob<double>*)sc.d_jobs;
+    if (R == 2)
+      hipLaunchKernelGGL((k_sigma_batch<double, 2>), dim3(blk), dim3(kSigBlock), shm, st, dj,
+                         (int)jobs.size());
+    else
+      hipLaunchKernelGGL((k_sigma_batch<double, 1>), dim3(blk), dim3(kSigBlock), shm, st, dj,
+                         (int)jobs.size());
+    if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
+    if (hipMemcpyAsync(sc.d_fin, sums.data(), sizeof(SumJob) * sums.size(), hipMemcpyHostToDevice,
+                       st) != hipSuccess)
+      return SBD_ERR_DEVICE;
+    hipLaunchKernelGGL(k_sum_fixed_batch, dim3((unsigned)sums.size()), dim3(kSumBlock), 0, st,
+                       (const SumJob*)sc.d_fin);
+    if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
+    if (hipMemcpyAsync(sc.h_out, sc.d_out, sizeof(double) * sel.size(), hipMemcpyDeviceToHost,
+                       st) != hipSuccess)
+      return SBD_ERR_DEVICE;
+    if (hipStreamSynchronize(st) != hipSuccess) return SBD_ERR_DEVICE;
+    for (size_t q = 0; q < sel.size(); q++) tw_out[sel[q]] = sc.h_out[q];
+  }
+  return SBD_OK;
+}
+
 template <typename VT>
 int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st,
                  double* tw_dot) {
@@ -531,7 +685,7 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
   const size_t row_bytes = sizeof(VT) * ctx->nb;
   const bool lds = row_bytes <= (size_t)ctx->lds_max;
   // rows per workgroup: two when both fit LDS together (and there are two)
-  const int rows = (lds && 2 * row_bytes <= (size_t)ctx->lds_max && ctx->na >= 2) ? 2 : 1;
+  const int rows = lds ? sigma_rows_for(ctx, sizeof(VT)) : 1;
   const uint64_t groups = (ctx->na + rows - 1) / rows;
   const unsigned grid = (unsigned)(groups < (uint64_t)kSigMaxGrid ? groups : kSigMaxGrid);
   double* dpart = nullptr;
@@ -599,7 +753,9 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
                        (const void*)k_sigma_rows<float, float, true, true, 1>,
                        (const void*)k_sigma_rows<float, float, true, false, 1>,
                        (const void*)k_sigma_rows<float, float, true, true, 2>,
-                       (const void*)k_sigma_rows<float, float, true, false, 2>};
+                       (const void*)k_sigma_rows<float, float, true, false, 2>,
+                       (const void*)k_sigma_batch<double, 1>,
+                       (const void*)k_sigma_batch<double, 2>};
   for (const void* f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipGetLastError();

@@ -11,6 +11,7 @@
 // All sums are FP64 with per-thread Kahan compensation; every kernel runs a
 // fixed grid and the block partials are summed in a fixed order, so results
 // are deterministic run to run.
+#include <algorithm>
 #include <initializer_list>
 
 #include "sbd_internal.hpp"
@@ -39,9 +40,17 @@ struct Kahan {
   }
 };
 
+// The share of one pass that a workgroup computes: block bid of nblk (the
+// launch grid for one vector pass, or one job's range inside a batch).
+struct Grid {
+  uint32_t bid, nblk;
+};
+__device__ inline Grid launch_grid() { return Grid{blockIdx.x, gridDim.x}; }
+
 // Block-reduce m per-thread values (value j held in v[j]) and write partials.
 template <int MAXM>
-__device__ inline void block_partials(double (&v)[MAXM], int m, double* __restrict__ part) {
+__device__ inline void block_partials(Grid g, double (&v)[MAXM], int m,
+                                      double* __restrict__ part) {
   __shared__ double red[kVecBlock / 64][MAXM];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int j = 0; j < m; j++) {
@@ -53,7 +62,7 @@ __device__ inline void block_partials(double (&v)[MAXM], int m, double* __restri
   if (threadIdx.x < m) {
     double s = 0.0;
     for (int w = 0; w < kVecBlock / 64; w++) s += red[w][threadIdx.x];
-    part[blockIdx.x * m + threadIdx.x] = s;
+    part[g.bid * m + threadIdx.x] = s;
   }
 }
 
@@ -100,9 +109,9 @@ constexpr int chunk_of() {
 // W-wide chunks at i = W * (tid + j * nthreads) while they fit (if packed),
 // then single elements.
 template <typename VT, typename F, int P = chunk_of<VT>()>
-__device__ inline void for_chunks(uint64_t n, bool packed, F&& body) {
-  const uint64_t tid = blockIdx.x * (uint64_t)kVecBlock + threadIdx.x;
-  const uint64_t nth = (uint64_t)gridDim.x * kVecBlock;
+__device__ inline void for_chunks(Grid g, uint64_t n, bool packed, F&& body) {
+  const uint64_t tid = g.bid * (uint64_t)kVecBlock + threadIdx.x;
+  const uint64_t nth = (uint64_t)g.nblk * kVecBlock;
   const uint64_t np = packed ? n / P : 0;
   for (uint64_t q = tid; q < np; q += nth) body.template operator()<P>(q * P);
   for (uint64_t i = np * P + tid; i < n; i += nth) body.template operator()<1>(i);
@@ -113,7 +122,7 @@ __global__ __launch_bounds__(kVecBlock) void k_multidot(VecPtrs<VT> X, int m,
                                                         const VT* __restrict__ y, uint64_t n,
                                                         bool packed, double* __restrict__ part) {
   Kahan acc[MAXM];
-  for_chunks<VT>(n, packed, [&]<int W>(uint64_t i) {
+  for_chunks<VT>(launch_grid(), n, packed, [&]<int W>(uint64_t i) {
     const Chunk<VT, W> yv = ldc<W>(y, i);
 #pragma unroll
     for (int j = 0; j < MAXM; j++) {
@@ -127,7 +136,7 @@ __global__ __launch_bounds__(kVecBlock) void k_multidot(VecPtrs<VT> X, int m,
   double v[MAXM];
 #pragma unroll
   for (int j = 0; j < MAXM; j++) v[j] = acc[j].s;
-  block_partials<MAXM>(v, m, part);
+  block_partials<MAXM>(launch_grid(), v, m, part);
 }
 
 // One pass of the Davidson step after the Ritz problem:
@@ -137,12 +146,10 @@ __global__ __launch_bounds__(kVecBlock) void k_multidot(VecPtrs<VT> X, int m,
 // <HC_j, t> (t as stored); the last group gives the next projected-matrix
 // column by symmetry of H, so no separate overlap pass follows the sigma.
 template <typename VT, int MAXK>
-__global__ __launch_bounds__(kVecBlock) void k_ritz_precond(VecPtrs<VT> C, VecPtrs<VT> HC, int k,
-                                                            Coefs u, double E0,
-                                                            const VT* __restrict__ dii,
-                                                            VT* __restrict__ t, uint64_t n,
-                                                            bool packed,
-                                                            double* __restrict__ part) {
+__device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>& HC, int k,
+                                 const Coefs& u, double E0, const VT* __restrict__ dii,
+                                 VT* __restrict__ t, uint64_t n, bool packed,
+                                 double* __restrict__ part) {
   constexpr double eps_reg = 1.0e-12;
   // norms compensated; the 2k overlaps as plain FP64 per-thread sums (the
   // register budget of 2k compensated sums would cut the waves in flight)
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(kVecBlock) void k_ritz_precond(VecPtrs<VT> C, VecPt
     }
   };
   constexpr int P = (sizeof(VT) == 4 && MAXK > 8) ? 2 : chunk_of<VT>();
-  for_chunks<VT, decltype(body)&, P>(n, packed, body);
+  for_chunks<VT, decltype(body)&, P>(g, n, packed, body);
   double v[2 * MAXK + 3];
   v[0] = rr.s;
   v[1] = ww.s;
@@ -213,17 +220,26 @@ __global__ __launch_bounds__(kVecBlock) void k_ritz_precond(VecPtrs<VT> C, VecPt
     v[3 + j] = ot[j];
     v[3 + MAXK + j] = oh[j];
   }
-  block_partials<2 * MAXK + 3>(v, 2 * MAXK + 3, part);
+  block_partials<2 * MAXK + 3>(g, v, 2 * MAXK + 3, part);
+}
+
+template <typename VT, int MAXK>
+__global__ __launch_bounds__(kVecBlock) void k_ritz_precond(VecPtrs<VT> C, VecPtrs<VT> HC, int k,
+                                                            Coefs u, double E0,
+                                                            const VT* __restrict__ dii,
+                                                            VT* __restrict__ t, uint64_t n,
+                                                            bool packed,
+                                                            double* __restrict__ part) {
+  ritz_body<VT, MAXK>(launch_grid(), C, HC, k, u, E0, dii, t, n, packed, part);
 }
 
 // Classical Gram-Schmidt update t = scale * (t - sum o_j C_j); sums |t|^2.
 template <typename VT, int MAXK>
-__global__ __launch_bounds__(kVecBlock) void k_gs_update(VecPtrs<VT> C, int k, Coefs o,
-                                                         VT* __restrict__ t, double scale,
-                                                         uint64_t n, bool packed,
-                                                         double* __restrict__ part) {
+__device__ inline void gs_body(Grid g, const VecPtrs<VT>& C, int k, const Coefs& o,
+                               VT* __restrict__ t, double scale, uint64_t n, bool packed,
+                               double* __restrict__ part) {
   Kahan tt;
-  for_chunks<VT>(n, packed, [&]<int W>(uint64_t i) {
+  for_chunks<VT>(g, n, packed, [&]<int W>(uint64_t i) {
     const Chunk<VT, W> tv = ldc<W>(t, i);
     double x[W];
 #pragma unroll
@@ -244,7 +260,95 @@ __global__ __launch_bounds__(kVecBlock) void k_gs_update(VecPtrs<VT> C, int k, C
     for (int e = 0; e < W; e++) tt.add((double)xs.v[e] * (double)xs.v[e]);
   });
   double v[1] = {tt.s};
-  block_partials<1>(v, 1, part);
+  block_partials<1>(g, v, 1, part);
+}
+
+template <typename VT, int MAXK>
+__global__ __launch_bounds__(kVecBlock) void k_gs_update(VecPtrs<VT> C, int k, Coefs o,
+                                                         VT* __restrict__ t, double scale,
+                                                         uint64_t n, bool packed,
+                                                         double* __restrict__ part) {
+  gs_body<VT, MAXK>(launch_grid(), C, k, o, t, scale, n, packed, part);
+}
+
+// ---------------------------------------------------------------- batches
+// Several jobs' passes in one launch: job j owns workgroups [blk0, blk0 + nblk)
+// with nblk = the grid it would get alone, so its sums are the same as in a
+// launch of its own.
+
+template <typename VT>
+struct RitzJob {
+  VecPtrs<VT> C, HC;
+  Coefs u;
+  const VT* dii;
+  VT* t;
+  uint64_t n;
+  double E0;
+  int k, packed;
+  uint32_t blk0, nblk;
+  double* part;
+};
+template <typename VT>
+struct GsJob {
+  VecPtrs<VT> C;
+  Coefs o;
+  VT* t;
+  uint64_t n;
+  double scale;
+  int k, packed;
+  uint32_t blk0, nblk;
+  double* part;
+};
+struct FinJob {
+  const double* part;
+  double* out;
+  int nblk, m;
+};
+
+template <typename J>
+__device__ inline const J& find_job(const J* __restrict__ jobs, int njobs) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  return jobs[lo];
+}
+
+template <typename VT, int MAXK>
+__global__ __launch_bounds__(kVecBlock) void k_ritz_batch(const RitzJob<VT>* __restrict__ jobs,
+                                                          int njobs) {
+  const RitzJob<VT>& J = find_job(jobs, njobs);
+  ritz_body<VT, MAXK>(Grid{blockIdx.x - J.blk0, J.nblk}, J.C, J.HC, J.k, J.u, J.E0, J.dii, J.t,
+                      J.n, J.packed != 0, J.part);
+}
+
+template <typename VT, int MAXK>
+__global__ __launch_bounds__(kVecBlock) void k_gs_batch(const GsJob<VT>* __restrict__ jobs,
+                                                        int njobs) {
+  const GsJob<VT>& J = find_job(jobs, njobs);
+  gs_body<VT, MAXK>(Grid{blockIdx.x - J.blk0, J.nblk}, J.C, J.k, J.o, J.t, J.scale, J.n,
+                    J.packed != 0, J.part);
+}
+
+// finalize for several jobs: block (j, s) sums job j's partials of sum s with
+// the arithmetic of k_finalize
+__global__ __launch_bounds__(kVecBlock) void k_finalize_batch(const FinJob* __restrict__ jobs,
+                                                              int maxm) {
+  __shared__ double red[kVecBlock];
+  const FinJob& J = jobs[blockIdx.x / maxm];
+  const int j = blockIdx.x % maxm;
+  if (j >= J.m) return;  // uniform over the block
+  Kahan k;
+  for (int b = threadIdx.x; b < J.nblk; b += kVecBlock) k.add(J.part[b * J.m + j]);
+  red[threadIdx.x] = k.s;
+  __syncthreads();
+  for (int w = kVecBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J.out[j] = red[0];
 }
 
 // out = scale * sum u_j C_j; sums |out|^2.
@@ -254,7 +358,7 @@ __global__ __launch_bounds__(kVecBlock) void k_lincomb(VecPtrs<VT> C, int k, Coe
                                                        uint64_t n, bool packed,
                                                        double* __restrict__ part) {
   Kahan tt;
-  for_chunks<VT>(n, packed, [&]<int W>(uint64_t i) {
+  for_chunks<VT>(launch_grid(), n, packed, [&]<int W>(uint64_t i) {
     double x[W];
     const Chunk<VT, W> c0 = ldc<W>(C.p[0], i);
 #pragma unroll
@@ -275,7 +379,7 @@ __global__ __launch_bounds__(kVecBlock) void k_lincomb(VecPtrs<VT> C, int k, Coe
     for (int e = 0; e < W; e++) tt.add((double)xs.v[e] * (double)xs.v[e]);
   });
   double v[1] = {tt.s};
-  block_partials<1>(v, 1, part);
+  block_partials<1>(launch_grid(), v, 1, part);
 }
 
 template <typename VT>
@@ -442,6 +546,179 @@ int vec_convert(sbd_ctx* ctx, const VS* x, VD* y, uint64_t n, hipStream_t st) {
   SBD_HIP_TRY(ctx, hipGetLastError());
   return SBD_OK;
 }
+
+// ---------------------------------------------------------------- batch launchers
+
+int batch_reserve(BatchScratch& sc, size_t job_bytes, size_t parts, size_t outs) {
+  auto grow = [](void** p, size_t* cap, size_t need, bool pinned) -> bool {
+    if (*cap >= need) return true;
+    if (*p) {
+      if (pinned) (void)hipHostFree(*p);
+      else (void)hipFree(*p);
+    }
+    *p = nullptr;
+    *cap = 0;
+    const size_t n = need + need / 2 + 256;
+    if ((pinned ? hipHostMalloc(p, n) : hipMalloc(p, n)) != hipSuccess) return false;
+    *cap = n;
+    return true;
+  };
+  if (!grow(&sc.d_jobs, &sc.jobs_cap, job_bytes, false) ||
+      !grow((void**)&sc.d_part, &sc.part_cap, parts * sizeof(double), false) ||
+      !grow((void**)&sc.d_out, &sc.out_cap, outs * sizeof(double), false) ||
+      !grow((void**)&sc.h_out, &sc.hout_cap, outs * sizeof(double), true) ||
+      !grow(&sc.d_fin, &sc.fin_cap, job_bytes, false))
+    return SBD_ERR_MEMORY;
+  return SBD_OK;
+}
+
+void batch_free(BatchScratch& sc) {
+  if (sc.d_jobs) (void)hipFree(sc.d_jobs);
+  if (sc.d_part) (void)hipFree(sc.d_part);
+  if (sc.d_out) (void)hipFree(sc.d_out);
+  if (sc.h_out) (void)hipHostFree(sc.h_out);
+  if (sc.d_fin) (void)hipFree(sc.d_fin);
+  sc = BatchScratch{};
+}
+
+// partials of every job, then their fixed-order sums, copied to the host
+static int batch_finish(BatchScratch& sc, const std::vector<FinJob>& fin, int maxm,
+                        size_t nout, hipStream_t st) {
+  if (hipMemcpyAsync(sc.d_fin, fin.data(), sizeof(FinJob) * fin.size(), hipMemcpyHostToDevice,
+                     st) != hipSuccess)
+    return SBD_ERR_DEVICE;
+  hipLaunchKernelGGL(k_finalize_batch, dim3((unsigned)(fin.size() * maxm)), dim3(kVecBlock), 0, st,
+                     (const FinJob*)sc.d_fin, maxm);
+  if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
+  if (hipMemcpyAsync(sc.h_out, sc.d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, st) !=
+      hipSuccess)
+    return SBD_ERR_DEVICE;
+  if (hipStreamSynchronize(st) != hipSuccess) return SBD_ERR_DEVICE;
+  return SBD_OK;
+}
+
+template <typename VT>
+int vec_ritz_batch(BatchScratch& sc, const RitzArgs<VT>* a, int njobs, double* const* out,
+                   hipStream_t st) {
+  if (njobs < 1) return SBD_OK;
+  int maxk = 1;
+  for (int j = 0; j < njobs; j++) {
+    if (a[j].k < 1 || a[j].k > kMaxVecs) return SBD_ERR_ARG;
+    maxk = std::max(maxk, a[j].k);
+  }
+  const int K = size_class(maxk), m = 2 * K + 3;
+  std::vector<RitzJob<VT>> jobs(njobs);
+  std::vector<FinJob> fin(njobs);
+  size_t parts = 0;
+  uint32_t blk = 0;
+  for (int j = 0; j < njobs; j++) {
+    const RitzArgs<VT>& x = a[j];
+    RitzJob<VT>& J = jobs[j];
+    J.C = pack(x.C, x.k);
+    J.HC = pack(x.HC, x.k);
+    J.u = packc(x.u, x.k);
+    J.dii = x.dii;
+    J.t = x.t;
+    J.n = x.n;
+    J.E0 = x.E0;
+    J.k = x.k;
+    J.packed = aligned16(x.C, x.k) && aligned16(x.HC, x.k) && aligned16<VT>({x.dii, x.t});
+    J.nblk = (uint32_t)grid_for(x.n);
+    J.blk0 = blk;
+    blk += J.nblk;
+    parts += (size_t)J.nblk * m;
+  }
+  int rc = batch_reserve(sc, std::max(sizeof(RitzJob<VT>), sizeof(FinJob)) * njobs, parts,
+                         (size_t)njobs * m);
+  if (rc) return rc;
+  size_t off = 0;
+  for (int j = 0; j < njobs; j++) {
+    jobs[j].part = sc.d_part + off;
+    fin[j] = FinJob{sc.d_part + off, sc.d_out + (size_t)j * m, (int)jobs[j].nblk, m};
+    off += (size_t)jobs[j].nblk * m;
+  }
+  if (hipMemcpyAsync(sc.d_jobs, jobs.data(), sizeof(RitzJob<VT>) * njobs, hipMemcpyHostToDevice,
+                     st) != hipSuccess)
+    return SBD_ERR_DEVICE;
+  const RitzJob<VT>* dj = (const RitzJob<VT>*)sc.d_jobs;
+  switch (K) {
+    case 4: hipLaunchKernelGGL((k_ritz_batch<VT, 4>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 8: hipLaunchKernelGGL((k_ritz_batch<VT, 8>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 12: hipLaunchKernelGGL((k_ritz_batch<VT, 12>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 16: hipLaunchKernelGGL((k_ritz_batch<VT, 16>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 24: hipLaunchKernelGGL((k_ritz_batch<VT, 24>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    default: hipLaunchKernelGGL((k_ritz_batch<VT, kMaxVecs>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+  }
+  if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
+  rc = batch_finish(sc, fin, m, (size_t)njobs * m, st);
+  if (rc) return rc;
+  for (int j = 0; j < njobs; j++) {
+    const double* r = sc.h_out + (size_t)j * m;
+    const int k = a[j].k;
+    for (int q = 0; q < 3 + k; q++) out[j][q] = r[q];
+    for (int q = 0; q < k; q++) out[j][3 + k + q] = r[3 + K + q];
+  }
+  return SBD_OK;
+}
+
+template <typename VT>
+int vec_gs_batch(BatchScratch& sc, const GsArgs<VT>* a, int njobs, double* out_norm2,
+                 hipStream_t st) {
+  if (njobs < 1) return SBD_OK;
+  int maxk = 1;
+  for (int j = 0; j < njobs; j++) {
+    if (a[j].k < 1 || a[j].k > kMaxVecs) return SBD_ERR_ARG;
+    maxk = std::max(maxk, a[j].k);
+  }
+  const int K = size_class(maxk);
+  std::vector<GsJob<VT>> jobs(njobs);
+  std::vector<FinJob> fin(njobs);
+  uint32_t blk = 0;
+  for (int j = 0; j < njobs; j++) {
+    const GsArgs<VT>& x = a[j];
+    GsJob<VT>& J = jobs[j];
+    J.C = pack(x.C, x.k);
+    J.o = packc(x.o, x.k);
+    J.t = x.t;
+    J.n = x.n;
+    J.scale = x.scale;
+    J.k = x.k;
+    J.packed = aligned16(x.C, x.k) && aligned16<VT>({x.t});
+    J.nblk = (uint32_t)grid_for(x.n);
+    J.blk0 = blk;
+    blk += J.nblk;
+  }
+  int rc = batch_reserve(sc, std::max(sizeof(GsJob<VT>), sizeof(FinJob)) * njobs, blk, njobs);
+  if (rc) return rc;
+  size_t off = 0;
+  for (int j = 0; j < njobs; j++) {
+    jobs[j].part = sc.d_part + off;
+    fin[j] = FinJob{sc.d_part + off, sc.d_out + j, (int)jobs[j].nblk, 1};
+    off += jobs[j].nblk;
+  }
+  if (hipMemcpyAsync(sc.d_jobs, jobs.data(), sizeof(GsJob<VT>) * njobs, hipMemcpyHostToDevice,
+                     st) != hipSuccess)
+    return SBD_ERR_DEVICE;
+  const GsJob<VT>* dj = (const GsJob<VT>*)sc.d_jobs;
+  switch (K) {
+    case 4: hipLaunchKernelGGL((k_gs_batch<VT, 4>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 8: hipLaunchKernelGGL((k_gs_batch<VT, 8>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 12: hipLaunchKernelGGL((k_gs_batch<VT, 12>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 16: hipLaunchKernelGGL((k_gs_batch<VT, 16>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    case 24: hipLaunchKernelGGL((k_gs_batch<VT, 24>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+    default: hipLaunchKernelGGL((k_gs_batch<VT, kMaxVecs>), dim3(blk), dim3(kVecBlock), 0, st, dj, njobs); break;
+  }
+  if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
+  rc = batch_finish(sc, fin, 1, njobs, st);
+  if (rc) return rc;
+  for (int j = 0; j < njobs; j++) out_norm2[j] = sc.h_out[j];
+  return SBD_OK;
+}
+
+template int vec_ritz_batch<double>(BatchScratch&, const RitzArgs<double>*, int, double* const*,
+                                    hipStream_t);
+template int vec_gs_batch<double>(BatchScratch&, const GsArgs<double>*, int, double*,
+                                  hipStream_t);
 
 #define SBD_INST(VT)                                                                          \
   template int vec_multidot<VT>(sbd_ctx*, const VT* const*, int, const VT*, uint64_t, double*, \

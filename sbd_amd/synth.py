@@ -205,3 +205,31 @@ def write_fcidump(path: str, L: int, nelec: int, I0: float, h: np.ndarray, eri: 
                 if abs(h[i, j]) > tol:
                     f.write(f"{h[i, j]!r} {i + 1} {j + 1} 0 0\n")
         f.write(f"{I0!r} 0 0 0 0\n")
+
+
+def serving_requests(count: int, seed: int, norb=(16, 40), nelec=(4, 10), strings=(200, 8000),
+                     rate: float = 0.0):
+    """A stream of independent solve requests: norb uniform in [norb], n_alpha =
+    n_beta = n uniform in [nelec], alpha and beta string counts log-uniform in
+    [strings] (capped at C(norb, n)), random weight-n strings plus HF, integrals of
+    rank 2 norb from a per-request seed.  With rate > 0 each request carries a
+    Poisson arrival time (exponential gaps of mean 1/rate seconds) in "t"."""
+    rng = SplitMix64(seed)
+    out, t = [], 0.0
+    for q in range(count):
+        r = rng.draw(6)
+        u = (r >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+        L = norb[0] + int(r[0] % np.uint64(norb[1] - norb[0] + 1))
+        n = nelec[0] + int(r[1] % np.uint64(nelec[1] - nelec[0] + 1))
+        n = min(n, L - 1)
+        cap = math.comb(L, n)
+        lo, hi = math.log(strings[0]), math.log(strings[1])
+        ca = min(cap, int(round(math.exp(lo + (hi - lo) * u[2]))))
+        cb = min(cap, int(round(math.exp(lo + (hi - lo) * u[3]))))
+        s0 = int(r[4] >> np.uint64(1))
+        if rate > 0:
+            t += -math.log(max(u[5], 1e-300)) / rate
+        I0, h, eri = make_integrals(L, s0, 2 * L)
+        out.append(dict(id=q, norb=L, n=n, I0=I0, h=h, eri=eri, t=t,
+                        a=random_strings(L, n, ca, s0 + 1), b=random_strings(L, n, cb, s0 + 2)))
+    return out

@@ -1,0 +1,110 @@
+"""Serving engine (sbd_server_*): continuous batching of independent solves.
+
+The property the engine is built around: every solve computes the same way
+whatever else is in the batch (the same workgroups and the same fixed-order
+partial sums as a launch of its own), so its energy is bit-identical
+
+  * run alone on a server,
+  * run in a mix with other solves that are admitted and retired around it,
+  * and run by the plain solver MultTPB.diag (sbd_diag) with the same options.
+
+Small synthetic requests of uneven norb, electron count and string count.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[serving] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    torch.cuda.set_device(0)
+    return torch
+
+
+@pytest.fixture(scope="module")
+def requests_():
+    from sbd_amd import synth
+    return synth.serving_requests(8, 808, norb=(10, 16), nelec=(3, 5), strings=(40, 300))
+
+
+def opts():
+    # up to 480 steps: most of these sparse requests converge to 1e-8, a few do
+    # not; agreement is checked either way
+    import sbd_amd
+    return sbd_amd.SBD(max_it=60, max_nb=8, eps=1e-8, init=1, seed=99)
+
+
+def test_server_alone_mix_and_plain_agree(torch, requests_):
+    import sbd_amd
+    say("plain solves")
+    plain = []
+    for r in requests_:
+        m = sbd_amd.MultTPB(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"])
+        E, _, rep = m.diag(opts())
+        m.close()
+        plain.append((E, rep["steps"], rep["converged"]))
+    say("each request alone on a server")
+    alone = []
+    for r in requests_:
+        sv = sbd_amd.Server(16 << 30, max_active=4)
+        rid = sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"], sbd=opts())
+        alone.append(sv.wait(rid))
+        sv.close()
+    say("all requests in one mix, fewer slots than requests")
+    sv = sbd_amd.Server(16 << 30, max_active=3)
+    ids = [sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"], sbd=opts())
+           for r in requests_]
+    mix = [sv.wait(i) for i in ids]
+    st = sv.stats()
+    sv.close()
+    assert st["peak_active"] >= 2, st
+    assert sum(p[2] for p in plain) >= len(plain) // 2
+    for q, (p, a, m) in enumerate(zip(plain, alone, mix)):
+        assert a["converged"] == m["converged"] == p[2]
+        assert a["energy"] == p[0], (q, a["energy"], p[0])
+        assert m["energy"] == a["energy"], (q, m["energy"], a["energy"])
+        assert a["steps"] == m["steps"] == p[1], (q, a["steps"], m["steps"], p[1])
+        assert m["t_submit"] <= m["t_admit"] <= m["t_done"]
+
+
+def test_server_memory_admission(torch, requests_):
+    """With a budget that fits one request at a time the solves still all
+    finish (admitted one after another) and give the same energies."""
+    import sbd_amd
+    rs = requests_[:3]
+    sv = sbd_amd.Server(1, max_active=8)  # every request exceeds the budget alone
+    ids = [sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"], sbd=opts())
+           for r in rs]
+    res = [sv.wait(i) for i in ids]
+    st = sv.stats()
+    sv.close()
+    assert st["peak_active"] == 1, st
+    sv = sbd_amd.Server(16 << 30, max_active=8)
+    ids = [sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"], sbd=opts())
+           for r in rs]
+    res2 = [sv.wait(i) for i in ids]
+    sv.close()
+    for a, b in zip(res, res2):
+        assert a["energy"] == b["energy"]
+    # solves were serialised: each admitted after the previous finished
+    for p, q in zip(res, res[1:]):
+        assert q["t_admit"] >= p["t_done"]
+
+
+def test_server_rejects_bad_requests(torch, requests_):
+    import sbd_amd
+    r = requests_[0]
+    sv = sbd_amd.Server(16 << 30, max_active=2)
+    with pytest.raises(sbd_amd.SbdError):
+        sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"],
+                  sbd=sbd_amd.SBD(max_nb=0))
+    with pytest.raises(sbd_amd.SbdError):
+        sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"],
+                  sbd=sbd_amd.SBD(precision=1))
+    sv.close()
