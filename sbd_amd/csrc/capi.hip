@@ -522,7 +522,7 @@ sbd_ctx* sbd_ctx_create(int norb, double I0, const double* h1, const double* eri
 void sbd_ctx_destroy(sbd_ctx* ctx) {
   if (!ctx) return;
   void* ptrs[] = {ctx->d_a,       ctx->d_b,       ctx->d_I1,      ctx->d_eri,     ctx->d_J,
-                  ctx->d_K,       ctx->d_G,       ctx->ta.s_off,  ctx->ta.d_off,  ctx->ta.s_rec,
+                  ctx->d_K,       ctx->d_G,       ctx->d_Vab,     ctx->ta.s_off,  ctx->ta.d_off,  ctx->ta.s_rec,
                   ctx->ta.d_rec,  ctx->tb.s_off,  ctx->tb.d_off,  ctx->tb.s_rec,  ctx->tb.d_rec,
                   ctx->d_partials, ctx->d_results, ctx->ta.with_s, ctx->tb.with_s,
                   ctx->ta.ell_ket, ctx->ta.ell_vm, ctx->ta.ell_off,
@@ -549,6 +549,8 @@ int sbd_make_helpers(sbd_ctx* ctx) {
   if (ctx->tables_built) return SBD_OK;
   int rc = build_tables(ctx, nullptr);
   if (!rc) rc = build_single_cache(ctx, nullptr);
+  if (!rc) rc = build_abtable(ctx, nullptr);
+  if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) rc = SBD_ERR_DEVICE;
   if (rc) return rc;
   ctx->tables_built = true;
   return SBD_OK;

@@ -169,6 +169,7 @@ struct sbd_ctx {
   double* d_J = nullptr;    // L x L  (pp|qq)
   double* d_K = nullptr;    // L x L  (pq|qp)
   double* d_G = nullptr;    // 2 x L x L x 2L: single-excitation integral sums per spin
+  double* d_Vab = nullptr;  // L^4: alpha-single x beta-single integral parts (when both exist)
   sbdx::SpinTables ta, tb;
   bool tables_built = false;
   // Single-excitation matrix elements, precomputed once per context (same
@@ -204,6 +205,7 @@ struct sbd_ctx {
 namespace sbdx {
 int build_tables(sbd_ctx* ctx, hipStream_t st);
 int build_gtable(sbd_ctx* ctx, hipStream_t st);
+int build_abtable(sbd_ctx* ctx, hipStream_t st);
 int build_single_cache(sbd_ctx* ctx, hipStream_t st);
 int launch_hdiag(sbd_ctx* ctx, double* hii, hipStream_t st);
 int launch_hdiag_f32(sbd_ctx* ctx, float* hii, hipStream_t st);

@@ -122,8 +122,9 @@ struct SigmaArgs {
   const double* Ea;  // alpha single elements [q][ib] (or null)
   const double* Eb;  // beta single elements [ia][q] (or null)
   uint32_t Sb;       // number of beta single records
-  const double* G;   // [2][L][L][2L]
-  const double* I2;  // packed ERI
+  const double* G;    // [2][L][L][2L]
+  const double* I2;   // packed ERI
+  const double* Vab;  // [L][L][L][L] alpha-single x beta-single integral parts
 };
 
 template <typename VT>
@@ -172,16 +173,20 @@ __device__ inline int popc_between(uint64_t D0, uint64_t D1, int lo, int hi) {
   return c;
 }
 
-// TwoExcite (determinants.h:500-518) on the full determinant.
-__device__ inline double two_excite_full(uint64_t D0, uint64_t D1, int i, int j, int a, int b,
-                                         const double* __restrict__ I2) {
+// Alpha-single x beta-single element TwoExcite(D, 2xa, 2xb+1, 2ya, 2yb+1)
+// (determinants.h:500-518): the parity of D over (I, A) and (J, B) times the
+// tabulated integral part (which carries the order-dependent sign): the same
+// operations as TwoExcite, so the same value bit for bit.
+__device__ inline double ab_elem(uint64_t D0, uint64_t D1, int xa, int ya, int xb, int yb, int L,
+                                 const double* __restrict__ Vab) {
+  const int i = 2 * xa, j = 2 * xb + 1, a = 2 * ya, b = 2 * yb + 1;
   const int I = i < j ? i : j, J = i < j ? j : i;
   const int A = a < b ? a : b, B = a < b ? b : a;
   int par = popc_between(D0, D1, I < A ? I : A, I < A ? A : I);
   par += popc_between(D0, D1, J < B ? J : B, J < B ? B : J);
-  double sgn = (par & 1) ? -1.0 : 1.0;
-  if (A > J || B < I) sgn *= -1.0;
-  return sgn * (eri_value(I2, A, I, B, J) - eri_value(I2, A, J, B, I));
+  const double v = Vab[(((int64_t)xa * L + ya) * L + xb) * L + yb];
+  const double sgn = (par & 1) ? -1.0 : 1.0;
+  return sgn * v;
 }
 
 // Single-excitation elements with their full sign (OneExcite, determinants.h:
@@ -234,6 +239,7 @@ constexpr int kSigBlock = kColBlock;  // 16 waves: one workgroup per CU when the
 constexpr unsigned kSigMaxGrid = 65536;  // rows beyond this are taken grid-stride
 constexpr int kBatch = 8;        // alpha pairs whose ket loads are issued together
 constexpr int kBBatch = 4;       // beta double slots fetched together
+constexpr int kABatch = 4;       // beta singles per alpha single fetched together (alpha-beta)
 
 template <typename VT>
 __device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ Trow, uint32_t nb) {
@@ -317,27 +323,21 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
         if (r < nr) stage_row<VT>(trows + (size_t)r * nb, p.T + (ia0 + r) * nb, nb);
       __syncthreads();
     }
-    uint64_t A[kRows];
-    uint32_t as0[kRows], as1[kRows], ad0[kRows], ad1[kRows];
-#pragma unroll
-    for (int r = 0; r < kRows; r++) {
-      const uint64_t ia = ia0 + (r < nr ? r : 0);
-      A[r] = p.a[ia];
-      as0[r] = p.as_off[ia];
-      as1[r] = p.as_off[ia + 1];
-      ad0[r] = p.ad_off[ia];
-      ad1[r] = p.ad_off[ia + 1];
-    }
+    // per-row values are re-read where they are used (workgroup-uniform scalar
+    // loads) rather than held for all rows, which would not fit the registers
+    // with several rows per workgroup
     for (uint32_t g = 0; g < nblk; g++) {
       const uint32_t ib = g * kSigBlock + threadIdx.x;
       // ---- phase 1: column ib of each row
       if (ib < nb) {
         const uint64_t B = p.b[ib];
         const uint32_t bs0 = p.bs_off[ib], bs1 = p.bs_off[ib + 1];
-#pragma unroll
-        for (int r = 0; r < kRows; r++) {
-          if (r >= nr) continue;
+#pragma unroll 1
+        for (int r = 0; r < nr; r++) {
           const uint64_t ia = ia0 + r;
+          const uint64_t A = p.a[ia];
+          const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
+          const uint32_t ad0 = p.ad_off[ia], ad1 = p.ad_off[ia + 1];
           const VT* __restrict__ Tsrc = kRowLds ? trows + (size_t)r * nb : p.T + ia * nb;
           VT* __restrict__ Wrow = p.W + ia * nb;
           const VT hv = p.hii[ia * nb + ib];
@@ -345,21 +345,21 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
           AT acc = p.accumulate ? (AT)Wrow[ib] : (AT)0;
           acc += (AT)hv * (AT)ts;
           // alpha singles (task type 2, OneExcite): workgroup-uniform records
-          for (uint32_t q = as0[r]; q < as1[r]; q++) {
+          for (uint32_t q = as0; q < as1; q++) {
             const PairRec rc = p.as_rec[q];
             const VT tk = p.T[(uint64_t)rc.ket * nb + ib];
             const double e = kCache ? p.Ea[(uint64_t)q * nb + ib]
-                                    : alpha_single_elem(rc, A[r], B, L, p.G);
+                                    : alpha_single_elem(rc, A, B, L, p.G);
             acc += (AT)e * (AT)tk;
           }
           // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
-          for (uint32_t q0 = ad0[r]; q0 < ad1[r]; q0 += kBatch) {
+          for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
             VT tv[kBatch];
             double vv[kBatch];
             uint64_t mm[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; u++) {
-              if (q0 + u < ad1[r]) {
+              if (q0 + u < ad1) {
                 const PairRec rc = p.ad_rec[q0 + u];
                 tv[u] = p.T[(uint64_t)rc.ket * nb + ib];
                 vv[u] = rc.val;
@@ -368,7 +368,7 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
             }
 #pragma unroll
             for (int u = 0; u < kBatch; u++) {
-              if (q0 + u < ad1[r]) {
+              if (q0 + u < ad1) {
                 const double e = (__popcll(B & mm[u]) & 1) ? -vv[u] : vv[u];
                 acc += (AT)e * (AT)tv[u];
               }
@@ -378,7 +378,7 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
           for (uint32_t q = bs0; q < bs1; q++) {
             const PairRec rc = p.bs_rec[q];
             const double e =
-                kCache ? p.Eb[ia * (uint64_t)p.Sb + q] : beta_single_elem(rc, A[r], B, L, p.G);
+                kCache ? p.Eb[ia * (uint64_t)p.Sb + q] : beta_single_elem(rc, A, B, L, p.G);
             acc += (AT)e * (AT)Tsrc[rc.ket];
           }
           Wrow[ib] = (VT)acc;
@@ -413,12 +413,13 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
             if (r >= nr) continue;
             const VT* __restrict__ Tsrc =
                 kRowLds ? trows + (size_t)r * nb : p.T + (ia0 + r) * nb;
+            const uint64_t A = p.a[ia0 + r];
             VT tt[kBBatch];
 #pragma unroll
             for (int u = 0; u < kBBatch; u++) tt[u] = Tsrc[kk[u]];
 #pragma unroll
             for (int u = 0; u < kBBatch; u++) {
-              const VT e = (__popcll(A[r] & mm[u]) & 1) ? -vv[u] : vv[u];
+              const VT e = (__popcll(A & mm[u]) & 1) ? -vv[u] : vv[u];
               const AT nv = acc[r] + (AT)e * (AT)tt[u];
               acc[r] = (k0 + u < cnt) ? nv : acc[r];
             }
@@ -429,21 +430,37 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
           if (r >= nr) continue;
           const uint64_t ia = ia0 + r;
           // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by))
-          if (as1[r] > as0[r]) {
+          const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
+          if (as1 > as0) {
             const uint32_t bs0 = p.bs_off[c], bs1 = p.bs_off[c + 1];
             if (bs1 > bs0) {
+              const uint64_t A = p.a[ia];
               const uint64_t B = p.b[c];
-              const uint64_t D0 = spread32(A[r]) | (spread32(B) << 1);
-              const uint64_t D1 = spread32(A[r] >> 32) | (spread32(B >> 32) << 1);
-              for (uint32_t q = as0[r]; q < as1[r]; q++) {
+              const uint64_t D0 = spread32(A) | (spread32(B) << 1);
+              const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
+              for (uint32_t q = as0; q < as1; q++) {
                 const PairRec ra = p.as_rec[q];
-                const int ai = 2 * code_x0(ra.code), aa = 2 * code_y0(ra.code);
+                const int xa = code_x0(ra.code), ya = code_y0(ra.code);
                 const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
-                for (uint32_t k = bs0; k < bs1; k++) {
-                  const PairRec rb = p.bs_rec[k];
-                  const int bi = 2 * code_x0(rb.code) + 1, ba = 2 * code_y0(rb.code) + 1;
-                  const double e = two_excite_full(D0, D1, ai, bi, aa, ba, p.I2);
-                  acc[r] += (AT)e * (AT)Tk[rb.ket];
+                // beta records kABatch at a time: their ket and integral loads
+                // are issued together, the sums are added in order (slots
+                // past the end repeat the last record and are discarded)
+                for (uint32_t k0 = bs0; k0 < bs1; k0 += kABatch) {
+                  uint32_t code[kABatch];
+                  VT tv[kABatch];
+#pragma unroll
+                  for (int u = 0; u < kABatch; u++) {
+                    const PairRec rb = p.bs_rec[k0 + u < bs1 ? k0 + u : bs1 - 1];
+                    code[u] = rb.code;
+                    tv[u] = Tk[rb.ket];
+                  }
+#pragma unroll
+                  for (int u = 0; u < kABatch; u++) {
+                    const double e =
+                        ab_elem(D0, D1, xa, ya, code_x0(code[u]), code_y0(code[u]), L, p.Vab);
+                    const AT nv = acc[r] + (AT)e * (AT)tv[u];
+                    acc[r] = (k0 + u < bs1) ? nv : acc[r];
+                  }
                 }
               }
             }
@@ -475,12 +492,12 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint32_t* __restrict__ bd_ell_off_, const uint32_t* __restrict__ bd_ell_col_,
     const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
     const double* __restrict__ G_, const double* __restrict__ I2_,
-    double* __restrict__ dot_part) {
+    const double* __restrict__ Vab_, double* __restrict__ dot_part) {
   const SigmaArgs<VT> p{a_,      b_,          na_,         nb_,         L_,          hii_,
                         T_,      W_,          accumulate_, as_off_,     as_rec_,     ad_off_,
                         ad_rec_, bs_off_,     bs_rec_,     bd_off_,     bd_ell_ket_, bd_ell_vm_,
                         bd_ell_off_, bd_ell_col_, Ea_,     Eb_,         Sb_,         G_,
-                        I2_};
+                        I2_,     Vab_};
   sigma_rows_body<VT, AT, kRowLds, kCache, kRows>(p, bd_ell_f32_, blockIdx.x, gridDim.x,
                                                   dot_part);
 }
@@ -581,29 +598,33 @@ static SigmaArgs<double> sigma_args(const sbd_ctx* ctx, const double* hii, const
                            T,         W,          0,          A.s_off,   A.s_rec,   A.d_off,
                            A.d_rec,   B.s_off,    B.s_rec,    B.d_off,   B.ell_ket, B.ell_vm,
                            B.ell_off, B.ell_col,  ctx->d_Ea,  ctx->d_Eb, (uint32_t)B.n_s,
-                           ctx->d_G,  ctx->d_eri};
+                           ctx->d_G,  ctx->d_eri, ctx->d_Vab};
 }
 
 // rows per workgroup for one context (a property of the context alone, so a
 // job computes the same way in any batch)
+// (the most rows that fit LDS together and that the basis has: up to 4 FP64
+// rows, whose per-row state still fits the registers, or 8 FP32 rows)
 static int sigma_rows_for(const sbd_ctx* ctx, size_t elem) {
   const size_t row_bytes = elem * ctx->nb;
-  return (2 * row_bytes <= (size_t)ctx->lds_max && ctx->na >= 2) ? 2 : 1;
+  int r = elem >= 8 ? 4 : 8;
+  while (r > 1 && (r * row_bytes > (size_t)ctx->lds_max || ctx->na < (uint64_t)r)) r /= 2;
+  return r;
 }
 
 int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* tw_out,
                        hipStream_t st) {
   if (n < 1) return SBD_OK;
-  std::vector<int> direct, rows1, rows2;
+  std::vector<int> direct, byrows[3];  // rows 1, 2, 4
   for (int j = 0; j < n; j++) {
     const sbd_ctx* c = items[j].ctx;
     if (!c->tables_built) return SBD_ERR_STATE;
     const bool cache = (c->ta.n_s == 0 || c->d_Ea) && (c->tb.n_s == 0 || c->d_Eb);
     const bool lds = sizeof(double) * c->nb <= (size_t)c->lds_max;
-    const uint64_t groups = (c->na + 1) / 2;
+    const int r = lds ? sigma_rows_for(c, sizeof(double)) : 1;
+    const uint64_t groups = (c->na + r - 1) / r;
     if (!cache || !lds || groups > kSigMaxGrid) direct.push_back(j);
-    else if (sigma_rows_for(c, sizeof(double)) == 2) rows2.push_back(j);
-    else rows1.push_back(j);
+    else byrows[r == 4 ? 2 : r == 2 ? 1 : 0].push_back(j);
   }
   // jobs outside the batched form: their own launch (same row groups and sums)
   for (int j : direct) {
@@ -611,10 +632,10 @@ int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* 
                                   &tw_out[j]);
     if (rc) return rc;
   }
-  for (int pass = 0; pass < 2; pass++) {
-    const std::vector<int>& sel = pass ? rows2 : rows1;
+  for (int pass = 0; pass < 3; pass++) {
+    const std::vector<int>& sel = byrows[pass];
     if (sel.empty()) continue;
-    const int R = pass ? 2 : 1;
+    const int R = 1 << pass;
     std::vector<SigmaJob<double>> jobs(sel.size());
     std::vector<SumJob> sums(sel.size());
     size_t parts = 0, shm = 0;
@@ -643,12 +664,11 @@ int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* 
                        hipMemcpyHostToDevice, st) != hipSuccess)
       return SBD_ERR_DEVICE;
     const SigmaJob<double>* dj = (const SigmaJob<double>*)sc.d_jobs;
-    if (R == 2)
-      hipLaunchKernelGGL((k_sigma_batch<double, 2>), dim3(blk), dim3(kSigBlock), shm, st, dj,
-                         (int)jobs.size());
-    else
-      hipLaunchKernelGGL((k_sigma_batch<double, 1>), dim3(blk), dim3(kSigBlock), shm, st, dj,
-                         (int)jobs.size());
+    switch (R) {
+      case 4: hipLaunchKernelGGL((k_sigma_batch<double, 4>), dim3(blk), dim3(kSigBlock), shm, st, dj, (int)jobs.size()); break;
+      case 2: hipLaunchKernelGGL((k_sigma_batch<double, 2>), dim3(blk), dim3(kSigBlock), shm, st, dj, (int)jobs.size()); break;
+      default: hipLaunchKernelGGL((k_sigma_batch<double, 1>), dim3(blk), dim3(kSigBlock), shm, st, dj, (int)jobs.size()); break;
+    }
     if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
     if (hipMemcpyAsync(sc.d_fin, sums.data(), sizeof(SumJob) * sums.size(), hipMemcpyHostToDevice,
                        st) != hipSuccess)
@@ -684,7 +704,7 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
   }
   const size_t row_bytes = sizeof(VT) * ctx->nb;
   const bool lds = row_bytes <= (size_t)ctx->lds_max;
-  // rows per workgroup: two when both fit LDS together (and there are two)
+  // rows per workgroup: as many as fit LDS together (up to 8)
   const int rows = lds ? sigma_rows_for(ctx, sizeof(VT)) : 1;
   const uint64_t groups = (ctx->na + rows - 1) / rows;
   const unsigned grid = (unsigned)(groups < (uint64_t)kSigMaxGrid ? groups : kSigMaxGrid);
@@ -707,13 +727,21 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
   ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,     \
       A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_vm, B.ell_f32, B.ell_off,   \
       B.ell_col, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,           \
-      (const double*)ctx->d_G, (const double*)ctx->d_eri, dpart
+      (const double*)ctx->d_G, (const double*)ctx->d_eri, (const double*)ctx->d_Vab, dpart
 #define SBD_LAUNCH(LDS, CACHE, ROWS)                                                            \
   hipLaunchKernelGGL((k_sigma_rows<VT, VT, LDS, CACHE, ROWS>), dim3(grid), dim3(kSigBlock), shm, \
                      st, SBD_SIGMA_ARGS)
   // precomputed single-excitation elements, when both caches exist
   const bool cache = (A.n_s == 0 || ctx->d_Ea) && (B.n_s == 0 || ctx->d_Eb);
-  if (lds && rows == 2) {
+  if (lds && rows == 8) {
+    if constexpr (sizeof(VT) == 4) {
+      if (cache) SBD_LAUNCH(true, true, 8);
+      else SBD_LAUNCH(true, false, 8);
+    }
+  } else if (lds && rows == 4) {
+    if (cache) SBD_LAUNCH(true, true, 4);
+    else SBD_LAUNCH(true, false, 4);
+  } else if (lds && rows == 2) {
     if (cache) SBD_LAUNCH(true, true, 2);
     else SBD_LAUNCH(true, false, 2);
   } else if (lds) {
@@ -746,16 +774,24 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
   int lds = 65536;
   SBD_HIP_TRY(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                          ctx->device));
-  const void* fns[] = {(const void*)k_sigma_rows<double, double, true, true, 1>,
-                       (const void*)k_sigma_rows<double, double, true, false, 1>,
-                       (const void*)k_sigma_rows<double, double, true, true, 2>,
-                       (const void*)k_sigma_rows<double, double, true, false, 2>,
-                       (const void*)k_sigma_rows<float, float, true, true, 1>,
-                       (const void*)k_sigma_rows<float, float, true, false, 1>,
-                       (const void*)k_sigma_rows<float, float, true, true, 2>,
-                       (const void*)k_sigma_rows<float, float, true, false, 2>,
-                       (const void*)k_sigma_batch<double, 1>,
-                       (const void*)k_sigma_batch<double, 2>};
+  const void* fns[] = {
+      (const void*)k_sigma_rows<double, double, true, true, 1>,
+      (const void*)k_sigma_rows<double, double, true, false, 1>,
+      (const void*)k_sigma_rows<double, double, true, true, 2>,
+      (const void*)k_sigma_rows<double, double, true, false, 2>,
+      (const void*)k_sigma_rows<double, double, true, true, 4>,
+      (const void*)k_sigma_rows<double, double, true, false, 4>,
+      (const void*)k_sigma_rows<float, float, true, true, 1>,
+      (const void*)k_sigma_rows<float, float, true, false, 1>,
+      (const void*)k_sigma_rows<float, float, true, true, 2>,
+      (const void*)k_sigma_rows<float, float, true, false, 2>,
+      (const void*)k_sigma_rows<float, float, true, true, 4>,
+      (const void*)k_sigma_rows<float, float, true, false, 4>,
+      (const void*)k_sigma_rows<float, float, true, true, 8>,
+      (const void*)k_sigma_rows<float, float, true, false, 8>,
+      (const void*)k_sigma_batch<double, 1>,
+      (const void*)k_sigma_batch<double, 2>,
+      (const void*)k_sigma_batch<double, 4>};
   for (const void* f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipGetLastError();

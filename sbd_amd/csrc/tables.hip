@@ -161,6 +161,37 @@ __global__ void k_gtable(const double* __restrict__ I2, int L, double* __restric
   }
 }
 
+// Vab[((xa L + ya) L + xb) L + yb]: the integral part of TwoExcite(D, i, j, a, b)
+// (determinants.h:500-518) for i = 2xa, a = 2ya (alpha) and j = 2xb+1, b = 2yb+1
+// (beta), with the sign that depends only on the order of the four spin
+// orbitals (-1 if A > J or B < I) applied: ((AI|BJ) - (AJ|BI)) negated or not,
+// by the same operations.  The sigma multiplies it by the parity sign only.
+__global__ void k_abtable(const double* __restrict__ I2, int L, double* __restrict__ V) {
+  const int64_t total = (int64_t)L * L * L * L;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int yb = (int)(t % L), xb = (int)(t / L % L), ya = (int)(t / L / L % L),
+              xa = (int)(t / L / L / L);
+    const int i = 2 * xa, j = 2 * xb + 1, a = 2 * ya, b = 2 * yb + 1;
+    const int I = i < j ? i : j, J = i < j ? j : i;
+    const int A = a < b ? a : b, B = a < b ? b : a;
+    double sgn = 1.0;
+    if (A > J || B < I) sgn *= -1.0;
+    V[t] = sgn * (eri_value(I2, A, I, B, J) - eri_value(I2, A, J, B, I));
+  }
+}
+
+int build_abtable(sbd_ctx* ctx, hipStream_t st) {
+  if (ctx->d_Vab || ctx->ta.n_s == 0 || ctx->tb.n_s == 0) return SBD_OK;
+  const int L = ctx->norb;
+  const size_t total = (size_t)L * L * L * L;
+  SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_Vab, sizeof(double) * total));
+  const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 65535);
+  hipLaunchKernelGGL(k_abtable, dim3(grid), dim3(256), 0, st, ctx->d_eri, L, ctx->d_Vab);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  return SBD_OK;
+}
+
 static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, SpinTables& T,
                       hipStream_t st) {
   uint32_t *sc = nullptr, *dc = nullptr;
