@@ -4,12 +4,15 @@
 // Requests arrive at any time (sbd_server_submit).  An admission thread takes
 // them in arrival order, reserves their device footprint against the budget,
 // builds their context (tables, diagonal, start vector, Davidson buffers) and
-// hands them to the engine.  The engine thread advances every admitted solve
-// one Davidson pass at a time (DavidsonJob, davidson_job.hpp) and runs each
-// kind of pass for all solves that need it in one launch: one sigma launch
-// (k_sigma_batch), one Ritz launch, one Gram-Schmidt launch per round.  A solve
-// that finishes frees its memory at once, and requests that arrived meanwhile
-// are admitted as memory and slots allow, so their work joins the next round.
+// hands each to one of a few engine lanes (a thread with its own stream),
+// balancing the lanes by estimated work per step so that light solves do not
+// wait each round for a heavy one.  A lane advances every solve it holds one
+// Davidson pass at a time (DavidsonJob, davidson_job.hpp) and runs each kind of
+// pass for all of them in one launch: one sigma launch (k_sigma_batch), one
+// Ritz launch, one Gram-Schmidt launch per round; the lanes' launches overlap
+// on the device.  A solve that finishes frees its memory at once, and requests
+// that arrived meanwhile are admitted as memory and slots allow, so their work
+// joins the next round.
 //
 // Each solve takes the same number of workgroups in a batch as it would
 // alone and sums its own partials in a fixed order, so its iterates -- and its
@@ -36,6 +39,8 @@ using namespace sbdx;
 
 namespace {
 
+constexpr int kServerLanes = 8;  // engine lanes (threads + streams)
+
 double clock_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -60,6 +65,8 @@ struct Solve {
   double* hii = nullptr;
   DavidsonJob<double> job;
   double t_admit = 0.0;
+  double cost = 0.0;  // estimated work of one step (terms), for lane balancing
+  int lane = 0;
   int status = SBD_OK;
 
   void release() {
@@ -77,28 +84,65 @@ struct Solve {
   }
 };
 
-// Device bytes of one solve: 2 max_nb + 2 vectors (C, HC, spare, hii) plus
-// tables and integrals (bounded as in sbd_diag_footprint).
+// Device bytes of one solve before its tables exist: 2 max_nb + 2 vectors
+// (C, HC, spare, hii), integrals, and tables and caches estimated for random
+// strings (each possible excitation partner present with probability
+// strings / C(norb, n)).  Corrected to the exact figure once the tables are
+// built.
 uint64_t footprint(const Request& r) {
   const uint64_t N = (uint64_t)r.a.size() * r.b.size();
   const uint64_t nb = (uint64_t)r.opt.max_nb;
   const uint64_t L = (uint64_t)r.norb;
-  uint64_t bytes = (2 * nb + 2) * N * 8;
-  bytes += 8 * (4 * L * L + r.eri.size() + 2 * L * L + 4 * L * L * L);
-  auto recs = [&](const std::vector<uint64_t>& s) {
-    const uint64_t n = s.empty() ? 0 : (uint64_t)__builtin_popcountll(s[0]);
-    const uint64_t v1 = n * (L - n), v2 = n * (n - 1) / 2 * (L - n) * (L - n - 1) / 2;
-    return s.size() * std::min<uint64_t>(v1 + v2, s.size());
+  double bytes = (double)(2 * nb + 2) * N * 8;
+  bytes += 8.0 * (4 * L * L + r.eri.size() + 2 * L * L + 4 * L * L * L + L * L * L * L);
+  auto est = [&](const std::vector<uint64_t>& s, double& singles) {
+    const double n = s.empty() ? 0 : (double)__builtin_popcountll(s[0]);
+    const double v1 = n * (L - n), v2 = n * (n - 1) / 2 * (L - n) * (L - n - 1) / 2;
+    double comb = 1.0;
+    for (double k = 0; k < n; k++) comb = comb * (L - k) / (k + 1);
+    const double fill = std::min(1.0, (double)s.size() / comb);
+    singles = s.size() * v1 * fill;
+    return s.size() * (v1 + v2) * fill;
   };
-  // records, ELL copies (at most 2x padding, 36 bytes a slot) and caches
-  bytes += 64 * (recs(r.a) + recs(r.b)) + 16 * (r.a.size() + r.b.size() + 2);
-  const uint64_t singles = std::min<uint64_t>(
-      (uint64_t)(r.a.empty() ? 0 : __builtin_popcountll(r.a[0])) * L, r.a.size());
-  bytes += 8 * singles * (r.a.size() + r.b.size()) * 2;
+  double sa = 0, sb = 0;
+  const double recs = est(r.a, sa) + est(r.b, sb);
+  bytes += 60.0 * recs + 16.0 * (r.a.size() + r.b.size() + 2);
+  bytes += 8.0 * (sa * r.b.size() + r.a.size() * sb);
+  return (uint64_t)bytes;
+}
+
+// The same after the tables are built, from the actual counts.
+uint64_t footprint_built(const sbd_ctx* c, int max_nb) {
+  const uint64_t N = c->N, L = (uint64_t)c->norb;
+  uint64_t bytes = (2 * (uint64_t)max_nb + 2) * N * 8;
+  bytes += 8 * (4 * L * L + c->eri.size() + 2 * L * L + 4 * L * L * L);
+  if (c->d_Vab) bytes += 8 * L * L * L * L;
+  bytes += 16 * (c->ta.n_s + c->ta.n_d + c->tb.n_s + c->tb.n_d);  // records
+  bytes += 2 * 36 * (c->tb.n_d + c->nb);                          // ELL copies (bounded)
+  bytes += 8 * 8 * (c->na + c->nb + 2);                           // offsets, orders, strings
+  if (c->d_Ea) bytes += 8 * c->ta.n_s * c->nb;
+  if (c->d_Eb) bytes += 8 * c->na * c->tb.n_s;
   return bytes;
 }
 
 }  // namespace
+
+// One engine lane: its own thread, stream and scratch, advancing the solves
+// assigned to it.  Solves are spread over the lanes by estimated step cost so
+// that light solves do not wait every round for a heavy one.
+struct Lane {
+  std::deque<Solve*> ready;
+  hipStream_t st = nullptr;
+  sbdx::BatchScratch sc;
+  std::thread th;
+  double load = 0.0;
+  std::vector<double> costs;  // step cost of each solve the lane holds
+  double max_cost() const {
+    double m = 0.0;
+    for (double c : costs) m = std::max(m, c);
+    return m;
+  }
+};
 
 struct sbd_server {
   int device = 0;
@@ -107,15 +151,13 @@ struct sbd_server {
   std::mutex mu;
   std::condition_variable cv_admit, cv_engine, cv_done;
   std::deque<Request> pending;
-  std::deque<Solve*> ready;
-  int preparing = 0, running = 0;
+  std::vector<Lane> lanes;
+  int preparing = 0, running = 0, queued = 0;  // queued: prepared, not yet on a lane's batch
   std::map<uint64_t, sbd_result> results;
   std::map<uint64_t, int> state;  // 0 queued, 1 admitted, 2 done
   bool stopping = false;
   uint64_t next_id = 1;
-  std::thread admitter, engine;
-  hipStream_t st = nullptr;
-  BatchScratch sc;
+  std::thread admitter;
   double t0 = 0.0;
   uint64_t rounds = 0, peak_active = 0;
   std::string last_error;
@@ -135,6 +177,12 @@ struct sbd_server {
     s->release();
     std::lock_guard<std::mutex> g(mu);
     used -= s->req.bytes;
+    lanes[s->lane].load -= s->cost;
+    {
+      auto& cs = lanes[s->lane].costs;
+      auto it = std::find(cs.begin(), cs.end(), s->cost);
+      if (it != cs.end()) cs.erase(it);
+    }
     results[s->req.id] = r;
     state[s->req.id] = 2;
     running--;
@@ -187,6 +235,10 @@ struct sbd_server {
       return SBD_ERR_DEVICE;
     if (hipDeviceSynchronize() != hipSuccess) return SBD_ERR_DEVICE;
     s->job.init(s->C, s->HC, s->spare, s->hii, N, q.opt.max_it, q.opt.eps);
+    const sbd_ctx* c = s->ctx;
+    const double na = (double)c->na, nbv = (double)c->nb;
+    s->cost = (double)N * (1.0 + (c->ta.n_s + c->ta.n_d) / na + (c->tb.n_s + c->tb.n_d) / nbv +
+                           (c->ta.n_s / na) * (c->tb.n_s / nbv));
     return SBD_OK;
   }
 
@@ -200,8 +252,8 @@ struct sbd_server {
           if (stopping && pending.empty()) return true;
           if (pending.empty()) return false;
           const uint64_t need = pending.front().bytes;
-          const bool slot = running + preparing + (int)ready.size() < max_active;
-          const bool mem = used + need <= budget || (running + preparing + ready.size() == 0);
+          const bool slot = running + preparing + queued < max_active;
+          const bool mem = used + need <= budget || (running + preparing + queued == 0);
           return slot && mem;
         });
         if (pending.empty()) return;  // stopping
@@ -216,6 +268,13 @@ struct sbd_server {
       const int rc = prepare(s);
       std::unique_lock<std::mutex> g(mu);
       preparing--;
+      if (!rc) {
+        // replace the estimate by the actual footprint
+        const uint64_t actual = footprint_built(s->ctx, s->req.opt.max_nb);
+        used = used - s->req.bytes + actual;
+        s->req.bytes = actual;
+        cv_admit.notify_all();
+      }
       if (rc) {
         s->status = rc;
         last_error = s->ctx ? s->ctx->last_error : "request preparation failed";
@@ -224,38 +283,56 @@ struct sbd_server {
         finish(s, 0.0);
         continue;
       }
-      ready.push_back(s);
+      // A lane's round lasts about as long as its heaviest solve's step, so a
+      // solve goes to the lane where it raises that the least (ties: the
+      // lane with the least work): light solves stay away from heavy ones.
+      int best = 0;
+      double best_round = 0.0;
+      for (int l = 0; l < (int)lanes.size(); l++) {
+        const double r = std::max(lanes[l].max_cost(), s->cost);
+        if (l == 0 || r < best_round || (r == best_round && lanes[l].load < lanes[best].load)) {
+          best = l;
+          best_round = r;
+        }
+      }
+      s->lane = best;
+      lanes[best].load += s->cost;
+      lanes[best].costs.push_back(s->cost);
+      lanes[best].ready.push_back(s);
+      queued++;
       cv_engine.notify_all();
     }
   }
 
   // ---- engine: one round advances every active solve through its passes
-  void engine_loop() {
+  void engine_loop(int li) {
     (void)hipSetDevice(device);
+    Lane& ln = lanes[li];
     std::vector<Solve*> active;
     for (;;) {
       {
         std::unique_lock<std::mutex> g(mu);
         cv_engine.wait(g, [&] {
-          return !ready.empty() || !active.empty() ||
+          return !ln.ready.empty() || !active.empty() ||
                  (stopping && pending.empty() && preparing == 0);
         });
-        while (!ready.empty()) {
-          Solve* s = ready.front();
-          ready.pop_front();
+        while (!ln.ready.empty()) {
+          Solve* s = ln.ready.front();
+          ln.ready.pop_front();
           s->t_admit = now();
+          queued--;
           running++;
           active.push_back(s);
         }
         if (active.empty() && stopping && pending.empty() && preparing == 0) return;
+        peak_active = std::max<uint64_t>(peak_active, (uint64_t)running);
+        rounds++;
       }
-      peak_active = std::max<uint64_t>(peak_active, active.size());
-      rounds++;
-      round(active);
+      round(active, ln);
       // completed solves leave the batch
       std::vector<Solve*> keep;
       for (Solve* s : active) {
-        if (s->job.next() == DavidsonJob<double>::kDone || s->status) complete(s);
+        if (s->job.next() == DavidsonJob<double>::kDone || s->status) complete(s, ln.st);
         else keep.push_back(s);
       }
       active.swap(keep);
@@ -267,8 +344,10 @@ struct sbd_server {
     s->job.stop();
   }
 
-  void round(std::vector<Solve*>& active) {
+  void round(std::vector<Solve*>& active, Lane& ln) {
     using DJ = DavidsonJob<double>;
+    hipStream_t st = ln.st;
+    BatchScratch& sc = ln.sc;
     // sigma for every solve that needs one, in one launch
     {
       std::vector<SigmaItem> items;
@@ -371,7 +450,7 @@ struct sbd_server {
   }
 
   // final vector and E = <W, H W> with a fresh sigma, as sbd_diag does
-  void complete(Solve* s) {
+  void complete(Solve* s, hipStream_t st) {
     double E = 0.0;
     if (!s->status) {
       const uint64_t N = s->ctx->N;
@@ -405,14 +484,20 @@ sbd_server* sbd_server_create(uint64_t mem_budget, int max_active, int* status) 
   sv->budget = mem_budget;
   sv->max_active = max_active;
   sv->t0 = clock_s();
-  if (hipGetDevice(&sv->device) != hipSuccess ||
-      hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess) {
+  sv->lanes = std::vector<Lane>(std::min(kServerLanes, max_active));
+  bool ok = hipGetDevice(&sv->device) == hipSuccess;
+  for (Lane& l : sv->lanes)
+    ok = ok && hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) == hipSuccess;
+  if (!ok) {
+    for (Lane& l : sv->lanes)
+      if (l.st) (void)hipStreamDestroy(l.st);
     if (status) *status = SBD_ERR_DEVICE;
     delete sv;
     return nullptr;
   }
   sv->admitter = std::thread([sv] { sv->admit_loop(); });
-  sv->engine = std::thread([sv] { sv->engine_loop(); });
+  for (int l = 0; l < (int)sv->lanes.size(); l++)
+    sv->lanes[l].th = std::thread([sv, l] { sv->engine_loop(l); });
   if (status) *status = SBD_OK;
   return sv;
 }
@@ -485,9 +570,11 @@ void sbd_server_destroy(sbd_server* sv) {
     std::lock_guard<std::mutex> g(sv->mu);
     sv->cv_engine.notify_all();
   }
-  if (sv->engine.joinable()) sv->engine.join();
-  batch_free(sv->sc);
-  if (sv->st) (void)hipStreamDestroy(sv->st);
+  for (Lane& l : sv->lanes) {
+    if (l.th.joinable()) l.th.join();
+    batch_free(l.sc);
+    if (l.st) (void)hipStreamDestroy(l.st);
+  }
   delete sv;
 }
 
