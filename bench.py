@@ -15,6 +15,10 @@ Printed JSON line:
                FP64 correction, same workload
   cpu_baseline the CPU oracle (restatement of the reference's OpenMP sigma) on
                a bounded sample of bra rows of the same workload
+  serving      a Poisson stream of independent solves (norb 16-40, n 4-10,
+               200-8000 strings per spin) through the continuous-batching
+               server vs the same stream solved one at a time: p50/p99
+               latency and solves/s
 """
 import argparse
 import json
@@ -81,6 +85,84 @@ def cpu_baseline(w, m, T_dev, Wb_dev, hii_dev, target_s):
             "sample_bitexact_vs_gpu": exact}
 
 
+def pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
+
+
+def serving(args):
+    """Poisson stream of independent solves (norb 16-40, n 4-10, 200-8000 strings
+    per spin, per-request integrals), served with continuous batching, against the
+    same stream solved one request at a time with the plain solver."""
+    import threading
+    import sbd_amd
+    from sbd_amd import synth
+    t0 = time.time()
+    reqs = synth.serving_requests(args.serve_count, 4004, rate=args.serve_rate)
+    log(f"serving: {len(reqs)} requests generated in {time.time() - t0:.1f}s, "
+        f"N total {sum(len(r['a']) * len(r['b']) for r in reqs):.3e}")
+    opt = sbd_amd.SBD(max_it=args.serve_max_it, max_nb=10, eps=args.serve_eps, init=0)
+    # continuous batching
+    sv = sbd_amd.Server(args.serve_mem_gb << 30, max_active=args.serve_slots)
+    ids = [None] * len(reqs)
+    base = sv.now()
+
+    def feeder():
+        for q, r in enumerate(reqs):
+            dt = base + r["t"] - sv.now()
+            if dt > 0:
+                time.sleep(dt)
+            ids[q] = sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"],
+                               sbd=opt)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    res = []
+    for q, i in enumerate(ids):
+        res.append(sv.wait(i))
+        if (q + 1) % 64 == 0:
+            log(f"serving: batched, {q + 1} solves done")
+    st = sv.stats()
+    sv.close()
+    lat = [r["t_done"] - r["t_submit"] for r in res]
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    log(f"serving: batched p50 {pct(lat, 50):.3f}s p99 {pct(lat, 99):.3f}s "
+        f"{len(res) / span:.2f} solves/s peak active {st['peak_active']}")
+    # sequential plain solves of the same stream (queueing included)
+    seq_lat, seq_E = [], []
+    t_start = time.time()
+    for r in reqs:
+        now = time.time() - t_start
+        if now < r["t"]:
+            time.sleep(r["t"] - now)
+        m = sbd_amd.MultTPB(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"])
+        E, _, rep = m.diag(opt)
+        m.close()
+        seq_E.append(E)
+        seq_lat.append(time.time() - t_start - r["t"])
+        if len(seq_E) % 32 == 0:
+            log(f"serving: sequential, {len(seq_E)} solves done")
+    seq_span = time.time() - t_start - reqs[0]["t"]
+    log(f"serving: sequential p50 {pct(seq_lat, 50):.3f}s p99 {pct(seq_lat, 99):.3f}s "
+        f"{len(reqs) / seq_span:.2f} solves/s")
+    same = sum(1 for a, b in zip(res, seq_E) if a["energy"] == b)
+    return {
+        "requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
+        "options": {"max_it": args.serve_max_it, "max_nb": 10, "eps": args.serve_eps, "init": 0},
+        "batched": {"solves_per_s": round(len(res) / span, 3), "p50_s": round(pct(lat, 50), 4),
+                    "p99_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3),
+                    "peak_active": st["peak_active"], "rounds": st["rounds"],
+                    "converged": sum(r["converged"] for r in res)},
+        "sequential_plain": {"solves_per_s": round(len(reqs) / seq_span, 3),
+                             "p50_s": round(pct(seq_lat, 50), 4),
+                             "p99_s": round(pct(seq_lat, 99), 4),
+                             "makespan_s": round(seq_span, 3)},
+        "throughput_ratio": round((len(res) / span) / (len(reqs) / seq_span), 3),
+        "energies_bit_identical": f"{same}/{len(reqs)}",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +175,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--dav-restarts", type=int, default=60,
                     help="Davidson max_it (restarts of max_nb=10 steps each)")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=64,
+                    help="requests in the stream (the serving configuration names 256)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--serve-mem-gb", type=int, default=160)
+    ap.add_argument("--serve-eps", type=float, default=1e-6)
+    ap.add_argument("--serve-max-it", type=int, default=30)
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -227,6 +317,9 @@ def main():
     else:
         out["cpu_baseline"] = None
     m.close()
+    del T, Wb, hii
+    if not args.no_serving:
+        out["serving"] = serving(args)
     print(json.dumps(out), flush=True)
 
 
