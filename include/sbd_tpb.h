@@ -173,6 +173,22 @@ int sbd_davidson(sbd_ctx* ctx, const double* hii, double* W, const sbd_options* 
 int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out, double* hist,
              int hist_cap, sbd_report* rep);
 
+/* The nroots lowest states by block Davidson (Davidson-Liu, restarted GD+k
+ * style from the current and previous Ritz vectors): sbd::tpb::diag
+ * (chemistry/tpb/sbdiag.h:151-526) finds the lowest root only
+ * (chemistry/tpb/davidson.h:707-970); this is its multi-root form.  nroots is
+ * 1..13; max_nb is the largest subspace (raised to 3 nroots, at most 40), max_it the
+ * number of block iterations, eps the residual every root must reach; init 0
+ * starts from the determinants with the lowest diagonal, init 1 from seeded
+ * random vectors (seed, seed + 1, ...).  precision = 1 runs the FP32 block
+ * phase to fp32_eps first.  energies[nroots] = <W_r, H W_r> (fresh FP64
+ * sigma), residuals[nroots] (optional), W_out (optional, host, nroots * N,
+ * root-major).  Returns SBD_OK also when max_it ends the iterations first
+ * (rep->converged tells). */
+int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* energies,
+                   double* residuals, double* W_out, double* hist, int hist_cap,
+                   sbd_report* rep);
+
 /* Device bytes sbd_diag would allocate for these options (vectors + tables +
  * integrals), for admission control. */
 int sbd_diag_footprint(const sbd_ctx* ctx, const sbd_options* opt, uint64_t* bytes);

@@ -32,7 +32,7 @@ EXPORTS = [
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
     "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
-    "sbd_diag", "sbd_diag_footprint", "sbd_server_create", "sbd_server_submit",
+    "sbd_diag", "sbd_diag_roots", "sbd_diag_footprint", "sbd_server_create", "sbd_server_submit",
     "sbd_server_wait", "sbd_server_poll", "sbd_server_now", "sbd_server_stats",
     "sbd_server_destroy",
 ]
@@ -111,6 +111,8 @@ def lib():
     L.sbd_davidson.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_diag.argtypes = [vp, C.POINTER(Options), dp, dp, dp, C.c_int, C.POINTER(Report)]
     L.sbd_diag_footprint.argtypes = [vp, C.POINTER(Options), up]
+    L.sbd_diag_roots.argtypes = [vp, C.POINTER(Options), C.c_int, dp, dp, dp, dp, C.c_int,
+                                 C.POINTER(Report)]
     L.sbd_server_create.restype = vp
     L.sbd_server_create.argtypes = [C.c_uint64, C.c_int, ip]
     L.sbd_server_submit.argtypes = [vp, C.POINTER(Request), up]
@@ -351,6 +353,29 @@ class MultTPB:
         if history:
             out = out + (hist[:4 * min(rep.steps, cap)].reshape(-1, 4),)
         return out
+
+    def diag_roots(self, sbd: SBD, nroots: int, want_vectors=False, history=False):
+        """Block Davidson for the nroots lowest states: returns (energies,
+        residuals, report[, W (nroots, N)][, history rows (E0, max |R|, s, 32/64)])."""
+        o = sbd.options()
+        E = np.zeros(nroots)
+        res = np.zeros(nroots)
+        W = np.zeros((nroots, self.N)) if want_vectors else None
+        cap = (o.max_it + 1) * 2 if history else 0
+        H = np.zeros(4 * max(cap, 1))
+        rep = Report()
+        _check(lib().sbd_diag_roots(self._ctx, C.byref(o), int(nroots), _ptr(E, C.c_double),
+                                    _ptr(res, C.c_double),
+                                    _ptr(W, C.c_double) if W is not None else None,
+                                    _ptr(H, C.c_double) if history else None, cap,
+                                    C.byref(rep)), self._ctx)
+        out = [E, res, rep.as_dict()]
+        if want_vectors:
+            out.append(W)
+        if history:
+            h = H.reshape(-1, 4)
+            out.append(h[: int(np.count_nonzero(h[:, 3]))])
+        return tuple(out)
 
     def footprint(self, sbd: SBD):
         o = sbd.options()
