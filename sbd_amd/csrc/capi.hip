@@ -527,7 +527,7 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
                   ctx->d_partials, ctx->d_results, ctx->ta.with_s, ctx->tb.with_s,
                   ctx->ta.ell_ket, ctx->ta.ell_vm, ctx->ta.ell_off,
                   ctx->tb.ell_ket, ctx->tb.ell_vm, ctx->tb.ell_off, ctx->d_sdot,
-                  ctx->ta.ell_col, ctx->tb.ell_col, ctx->ta.ell_f32, ctx->tb.ell_f32,
+                  ctx->ta.ell_pos, ctx->tb.ell_pos, ctx->ta.ell_f32, ctx->tb.ell_f32,
                   ctx->ta.s_bra,   ctx->tb.s_bra,   ctx->d_Ea,       ctx->d_Eb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -120,6 +120,12 @@ struct alignas(16) EllF32 {
   uint64_t mask;
 };
 
+// Column (string) at one block-sorted ELL position and its doubles count.
+struct alignas(8) EllPos {
+  uint32_t col;
+  uint32_t cnt;
+};
+
 // One spin's excitation tables on the device (CSR by bra string).
 struct SpinTables {
   uint32_t* s_off = nullptr;  // n+1
@@ -133,7 +139,7 @@ struct SpinTables {
   // Doubles in sliced-ELL order for per-column access by a wavefront.  The
   // strings are taken in blocks of kColBlock; within a block, positions list
   // the block's strings in descending order of their doubles count (stable),
-  // ell_col[position] = string.  Slice s holds positions [64s, 64s+64);
+  // ell_pos[position] = {string, its doubles count}.  Slice s holds positions [64s, 64s+64);
   // record u of position 64s+l is at slot ell_off[s] + 64u + l for u below
   // that string's count (slots past the count repeat its last record).  Stored as two arrays: ket index, and value with
   // the other-spin sign mask (the element is -val when popcount(other & mask)
@@ -142,7 +148,7 @@ struct SpinTables {
   ValMask* ell_vm = nullptr;
   EllF32* ell_f32 = nullptr;  // the same slots for the FP32 sigma
   uint32_t* ell_off = nullptr;  // n_slices + 1
-  uint32_t* ell_col = nullptr;  // n: string at each block-sorted position
+  EllPos* ell_pos = nullptr;   // n: string and count at each block-sorted position
   uint32_t n_slices = 0;
   uint32_t* s_bra = nullptr;    // bra string of each single record
 };

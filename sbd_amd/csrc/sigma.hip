@@ -118,7 +118,7 @@ struct SigmaArgs {
   const uint32_t* bd_ell_ket;  // beta doubles, sliced-ELL order
   const ValMask* bd_ell_vm;
   const uint32_t* bd_ell_off;
-  const uint32_t* bd_ell_col;  // column at each block-sorted position
+  const EllPos* bd_ell_pos;    // column and its doubles count at each block-sorted position
   const double* Ea;  // alpha single elements [q][ib] (or null)
   const double* Eb;  // beta single elements [ia][q] (or null)
   uint32_t Sb;       // number of beta single records
@@ -387,8 +387,8 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
       __syncthreads();
       // ---- phase 2: the column at block-sorted position ib, every row
       if (ib < nb) {
-        const uint32_t c = p.bd_ell_col[ib];
-        const uint32_t bd0 = p.bd_off[c], cnt = p.bd_off[c + 1] - bd0;
+        const EllPos ps = p.bd_ell_pos[ib];
+        const uint32_t c = ps.col, cnt = ps.cnt;
         AT acc[kRows];
 #pragma unroll
         for (int r = 0; r < kRows; r++) acc[r] = r < nr ? (AT)p.W[(ia0 + r) * nb + c] : (AT)0;
@@ -489,14 +489,14 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint32_t* __restrict__ bs_off_, const PairRec* __restrict__ bs_rec_,
     const uint32_t* __restrict__ bd_off_, const uint32_t* __restrict__ bd_ell_ket_,
     const ValMask* __restrict__ bd_ell_vm_, const EllF32* __restrict__ bd_ell_f32_,
-    const uint32_t* __restrict__ bd_ell_off_, const uint32_t* __restrict__ bd_ell_col_,
+    const uint32_t* __restrict__ bd_ell_off_, const EllPos* __restrict__ bd_ell_pos_,
     const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
     const double* __restrict__ G_, const double* __restrict__ I2_,
     const double* __restrict__ Vab_, double* __restrict__ dot_part) {
   const SigmaArgs<VT> p{a_,      b_,          na_,         nb_,         L_,          hii_,
                         T_,      W_,          accumulate_, as_off_,     as_rec_,     ad_off_,
                         ad_rec_, bs_off_,     bs_rec_,     bd_off_,     bd_ell_ket_, bd_ell_vm_,
-                        bd_ell_off_, bd_ell_col_, Ea_,     Eb_,         Sb_,         G_,
+                        bd_ell_off_, bd_ell_pos_, Ea_,     Eb_,         Sb_,         G_,
                         I2_,     Vab_};
   sigma_rows_body<VT, AT, kRowLds, kCache, kRows>(p, bd_ell_f32_, blockIdx.x, gridDim.x,
                                                   dot_part);
@@ -597,7 +597,7 @@ static SigmaArgs<double> sigma_args(const sbd_ctx* ctx, const double* hii, const
   return SigmaArgs<double>{ctx->d_a,  ctx->d_b,   ctx->na,    ctx->nb,   ctx->norb, hii,
                            T,         W,          0,          A.s_off,   A.s_rec,   A.d_off,
                            A.d_rec,   B.s_off,    B.s_rec,    B.d_off,   B.ell_ket, B.ell_vm,
-                           B.ell_off, B.ell_col,  ctx->d_Ea,  ctx->d_Eb, (uint32_t)B.n_s,
+                           B.ell_off, B.ell_pos,  ctx->d_Ea,  ctx->d_Eb, (uint32_t)B.n_s,
                            ctx->d_G,  ctx->d_eri, ctx->d_Vab};
 }
 
@@ -726,7 +726,7 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
 #define SBD_SIGMA_ARGS                                                                          \
   ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,     \
       A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_vm, B.ell_f32, B.ell_off,   \
-      B.ell_col, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,           \
+      B.ell_pos, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,           \
       (const double*)ctx->d_G, (const double*)ctx->d_eri, (const double*)ctx->d_Vab, dpart
 #define SBD_LAUNCH(LDS, CACHE, ROWS)                                                            \
   hipLaunchKernelGGL((k_sigma_rows<VT, VT, LDS, CACHE, ROWS>), dim3(grid), dim3(kSigBlock), shm, \

@@ -112,12 +112,12 @@ __global__ __launch_bounds__(kTabBlock) void k_tab_fill(const uint64_t* __restri
 // code.
 __global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off,
                           const PairRec* __restrict__ rec, const uint32_t* __restrict__ ell_off,
-                          const uint32_t* __restrict__ ell_col, uint32_t* __restrict__ ell_ket,
+                          const EllPos* __restrict__ ell_pos, uint32_t* __restrict__ ell_ket,
                           ValMask* __restrict__ ell_vm, EllF32* __restrict__ ell_f32) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = i / kEllSlice, l = i % kEllSlice;
-  const uint32_t col = ell_col[i];
+  const uint32_t col = ell_pos[i].col;
   const uint32_t o0 = off[col], cnt = off[col + 1] - o0;
   const uint32_t w = (ell_off[s + 1] - ell_off[s]) / kEllSlice;
   for (uint32_t u = 0; u < w; u++) {
@@ -261,8 +261,10 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
     std::stable_sort(col.begin() + g0, col.begin() + g1,
                      [&](uint32_t x, uint32_t y) { return hd[x] > hd[y]; });
   }
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_col, sizeof(uint32_t) * n));
-  SBD_HIP_TRY(ctx, hipMemcpy(T.ell_col, col.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+  std::vector<EllPos> pos(n);
+  for (uint32_t i = 0; i < n; i++) pos[i] = EllPos{col[i], hd[col[i]]};
+  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_pos, sizeof(EllPos) * n));
+  SBD_HIP_TRY(ctx, hipMemcpy(T.ell_pos, pos.data(), sizeof(EllPos) * n, hipMemcpyHostToDevice));
   T.n_slices = (n + kEllSlice - 1) / kEllSlice;
   std::vector<uint32_t> eoff(T.n_slices + 1, 0);
   for (uint32_t s = 0; s < T.n_slices; s++) {
@@ -279,7 +281,7 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
   SBD_HIP_TRY(ctx, hipMalloc(&T.ell_vm, sizeof(ValMask) * nslot));
   SBD_HIP_TRY(ctx, hipMalloc(&T.ell_f32, sizeof(EllF32) * nslot));
   hipLaunchKernelGGL(k_tab_ell, dim3((n + 255) / 256), dim3(256), 0, st, n, spin, T.d_off,
-                     T.d_rec, T.ell_off, T.ell_col, T.ell_ket, T.ell_vm, T.ell_f32);
+                     T.d_rec, T.ell_off, T.ell_pos, T.ell_ket, T.ell_vm, T.ell_f32);
   SBD_HIP_TRY(ctx, hipGetLastError());
   // bra string of every single record
   std::vector<uint32_t> sb(tot_s + 1, 0);
