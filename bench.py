@@ -272,7 +272,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": B,
-                     "kernel": "k_sigma_rows<double, double, true>"},
+                     "kernel": "k_sigma_rows<double, true, true, 1, true>"},
         "setup_s": {"tables": round(t_tables, 3), "hdiag": round(t_hdiag, 3)},
     }
 

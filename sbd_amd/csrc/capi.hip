@@ -525,9 +525,9 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
                   ctx->d_K,       ctx->d_G,       ctx->d_Vab,     ctx->ta.s_off,  ctx->ta.d_off,  ctx->ta.s_rec,
                   ctx->ta.d_rec,  ctx->tb.s_off,  ctx->tb.d_off,  ctx->tb.s_rec,  ctx->tb.d_rec,
                   ctx->d_partials, ctx->d_results, ctx->ta.with_s, ctx->tb.with_s,
-                  ctx->ta.ell_ket, ctx->ta.ell_vm, ctx->ta.ell_off,
-                  ctx->tb.ell_ket, ctx->tb.ell_vm, ctx->tb.ell_off, ctx->d_sdot,
-                  ctx->ta.ell_pos, ctx->tb.ell_pos, ctx->ta.ell_f32, ctx->tb.ell_f32,
+                  ctx->ta.ell64, ctx->ta.ell32, ctx->ta.ell_off, ctx->ta.ell_ket,
+                  ctx->tb.ell64, ctx->tb.ell32, ctx->tb.ell_off, ctx->tb.ell_ket, ctx->d_sdot,
+                  ctx->ta.ell_pos, ctx->tb.ell_pos, ctx->ta.col_rec, ctx->tb.col_rec,
                   ctx->ta.s_bra,   ctx->tb.s_bra,   ctx->d_Ea,       ctx->d_Eb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -106,18 +106,30 @@ __host__ __device__ inline double eri_value(const double* I2, int i, int j, int 
   return I2[eri_index(i / 2, j / 2, k / 2, l / 2)];
 }
 
-// Value and other-spin sign mask of one doubles slot, fetched as one 16-byte load.
-struct alignas(16) ValMask {
+// One beta-double slot of the sliced ELL: the value and the alpha-string mask
+// of the element's sign (the element is -val when popcount(alpha & mask) is
+// odd).  When every alpha string fits below bit kEllKetShift and the ket
+// index fits the bits above it ("packed" tables), the ket string index rides
+// in those bits, so a slot is one 16-byte load (FP64) or one 12-byte load
+// (FP32, value rounded to FP32: the FP32 kernel rounds every element to FP32
+// before its product, so both give the same sum); otherwise the kets are a
+// separate array.  Padding slots hold value 0 and mask 0 (their product adds
+// an exact zero) with the position's own string as ket.
+constexpr int kEllKetShift = 40;
+struct alignas(16) EllSlot64 {
   double val;
-  uint64_t mask;
+  uint64_t mk;
+};
+struct alignas(4) EllSlot32 {
+  float val;
+  uint32_t mlo, mhi;
 };
 
-// FP32 copy of one doubles slot (ket, value rounded to FP32, sign mask): one
-// 16-byte load in the FP32 sigma.
-struct alignas(16) EllF32 {
-  uint32_t ket;
-  float val;
-  uint64_t mask;
+// Per-column record of the beta strings read by the sigma kernel in one load:
+// the string and the range [s0, s1) of its single-excitation records.
+struct alignas(16) ColRec {
+  uint64_t B;
+  uint32_t s0, s1;
 };
 
 // Column (string) at one block-sorted ELL position and its doubles count.
@@ -136,19 +148,21 @@ struct SpinTables {
   uint32_t max_s = 0, max_d = 0;  // max records per bra
   uint32_t* with_s = nullptr;     // ascending list of strings that have single excitations
   uint32_t n_with_s = 0;
-  // Doubles in sliced-ELL order for per-column access by a wavefront.  The
+  // Doubles in sliced-ELL order for per-column access by a wavefront (built
+  // for the beta tables only: alpha doubles are read by row from the CSR).  The
   // strings are taken in blocks of kColBlock; within a block, positions list
   // the block's strings in descending order of their doubles count (stable),
-  // ell_pos[position] = {string, its doubles count}.  Slice s holds positions [64s, 64s+64);
-  // record u of position 64s+l is at slot ell_off[s] + 64u + l for u below
-  // that string's count (slots past the count repeat its last record).  Stored as two arrays: ket index, and value with
-  // the other-spin sign mask (the element is -val when popcount(other & mask)
-  // is odd).
-  uint32_t* ell_ket = nullptr;
-  ValMask* ell_vm = nullptr;
-  EllF32* ell_f32 = nullptr;  // the same slots for the FP32 sigma
+  // ell_pos[position] = {string, its doubles count}.  Slice s holds positions
+  // [64s, 64s+64) and is as wide as its largest count; record u of position
+  // 64s+l is at slot ell_off[s] + 64u + l (zero padding past its count).
+  EllSlot64* ell64 = nullptr;
+  EllSlot32* ell32 = nullptr;   // the same slots for the FP32 sigma
+  uint32_t* ell_ket = nullptr;  // ket of each slot, when the tables are not packed
+  bool ell_packed = false;
+  uint64_t n_slots = 0;
   uint32_t* ell_off = nullptr;  // n_slices + 1
   EllPos* ell_pos = nullptr;   // n: string and count at each block-sorted position
+  ColRec* col_rec = nullptr;   // n: string and single-excitation range of each string
   uint32_t n_slices = 0;
   uint32_t* s_bra = nullptr;    // bra string of each single record
 };

@@ -115,10 +115,12 @@ struct SigmaArgs {
   const uint32_t* bs_off;
   const PairRec* bs_rec;
   const uint32_t* bd_off;
-  const uint32_t* bd_ell_ket;  // beta doubles, sliced-ELL order
-  const ValMask* bd_ell_vm;
+  const EllSlot64* bd_ell64;  // beta doubles, sliced-ELL order (FP64 slots)
+  const EllSlot32* bd_ell32;  // the same slots for the FP32 sigma
+  const uint32_t* bd_ell_ket; // kets of the slots (unpacked tables only)
   const uint32_t* bd_ell_off;
   const EllPos* bd_ell_pos;    // column and its doubles count at each block-sorted position
+  const ColRec* b_rec;         // beta string and single-excitation range per column
   const double* Ea;  // alpha single elements [q][ib] (or null)
   const double* Eb;  // beta single elements [ia][q] (or null)
   uint32_t Sb;       // number of beta single records
@@ -130,7 +132,6 @@ struct SigmaArgs {
 template <typename VT>
 struct SigmaJob {
   SigmaArgs<VT> p;
-  const EllF32* ell_f32;
   uint32_t blk0, ngrp;  // first workgroup and number of row groups
   double* dot_part;     // ngrp * 16 partials of <T, W>
 };
@@ -236,10 +237,15 @@ __global__ void k_single_beta(const uint64_t* __restrict__ a, const uint64_t* __
 }
 
 constexpr int kSigBlock = kColBlock;  // 16 waves: one workgroup per CU when the row fills LDS
+constexpr int kHalf = kSigBlock / 2;  // threads of each role (8 waves)
+constexpr int kSigCols = kColBlock / kHalf;  // columns (positions) a thread carries per block
 constexpr unsigned kSigMaxGrid = 65536;  // rows beyond this are taken grid-stride
 constexpr int kBatch = 8;        // alpha pairs whose ket loads are issued together
-constexpr int kBBatch = 4;       // beta double slots fetched together
 constexpr int kABatch = 4;       // beta singles per alpha single fetched together (alpha-beta)
+// beta double slots of one position fetched together: fewer when the partial
+// sums of several rows share the registers
+template <typename VT, int kRows>
+constexpr int kBBatch = sizeof(VT) * kRows >= 16 ? 2 : sizeof(VT) * kRows >= 8 ? 4 : 8;
 
 template <typename VT>
 __device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ Trow, uint32_t nb) {
@@ -255,64 +261,294 @@ __device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ T
   for (; j < nb; j += kSigBlock) trow[j] = Trow[j];
 }
 
-// Beta-double ELL slot as the kernel of element type VT reads it: FP64 from
-// the ket array and the 16-byte value/mask array, FP32 from one 16-byte record
-// whose value is the FP64 value rounded to FP32 (the FP32 kernel rounds the
-// element to FP32 before its product either way, so both give the same sum).
+// v, or -v when the low bit of par is set (the sign bit flipped: the same
+// value as negation)
+__device__ inline double flip_sign(double v, uint32_t par) {
+  return __hiloint2double(__double2hiint(v) ^ (int)(par << 31), __double2loint(v));
+}
+__device__ inline float flip_sign(float v, uint32_t par) {
+  return __int_as_float(__float_as_int(v) ^ (int)(par << 31));
+}
+
+// Beta-double ELL slot as the kernel of element type VT reads it: value, alpha
+// sign mask and ket (from the slot's upper bits when packed).
 template <typename VT>
-struct EllRead;
+struct EllOf;
 template <>
-struct EllRead<double> {
-  __device__ static inline void load(const uint32_t* __restrict__ ket,
-                                     const ValMask* __restrict__ vm, const EllF32* __restrict__,
-                                     size_t s, uint32_t& k, double& v, uint64_t& m) {
-    k = ket[s];
-    const ValMask x = vm[s];
-    v = x.val;
-    m = x.mask;
+struct EllOf<double> {
+  template <bool kPk>
+  __device__ static inline void load(const SigmaArgs<double>& p, uint32_t i, uint32_t& ket,
+                                     double& val, uint64_t& mask) {
+    const EllSlot64 s = p.bd_ell64[i];
+    val = s.val;
+    mask = s.mk;
+    ket = kPk ? (uint32_t)(s.mk >> kEllKetShift) : p.bd_ell_ket[i];
   }
 };
 template <>
-struct EllRead<float> {
-  __device__ static inline void load(const uint32_t* __restrict__, const ValMask* __restrict__,
-                                     const EllF32* __restrict__ f, size_t s, uint32_t& k, float& v,
-                                     uint64_t& m) {
-    const EllF32 x = f[s];
-    k = x.ket;
-    v = x.val;
-    m = x.mask;
+struct EllOf<float> {
+  template <bool kPk>
+  __device__ static inline void load(const SigmaArgs<float>& p, uint32_t i, uint32_t& ket,
+                                     float& val, uint64_t& mask) {
+    const EllSlot32 s = p.bd_ell32[i];
+    val = s.val;
+    mask = ((uint64_t)s.mhi << 32) | s.mlo;
+    ket = kPk ? (s.mhi >> (kEllKetShift - 32)) : p.bd_ell_ket[i];
   }
 };
 
+// Phase 1 of one 1024-column block for the rows of a group: thread t (of the
+// first half) owns columns blk + t + 512 j (records cr[j], loaded ahead):
+// diagonal, alpha singles, alpha doubles and beta singles, in that order; the
+// partial sums are stored to W.  Every access to T, hii and W is a coalesced
+// stream along the columns; the alpha records are workgroup-uniform.  Columns
+// past the end are clamped to the last one (loads stay in bounds; nothing is
+// stored).
+template <typename VT, bool kRowLds, bool kCache>
+__device__ inline void sigma_phase1(const SigmaArgs<VT>& p, const VT* __restrict__ trows,
+                                    const uint64_t* A, uint64_t ia0, int nr, uint32_t blk,
+                                    uint32_t t, const ColRec* cr) {
+  using AT = VT;
+  constexpr int C = kSigCols;
+  const uint32_t nb = (uint32_t)p.nb;
+  const int L = p.L;
+  uint32_t col[C];
+  bool ok[C];
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const uint32_t c = blk + j * kHalf + t;
+    ok[j] = c < nb;
+    col[j] = ok[j] ? c : nb - 1;
+  }
+#pragma unroll 1
+  for (int r = 0; r < nr; r++) {
+    const uint64_t ia = ia0 + r;
+    const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
+    const uint32_t ad0 = p.ad_off[ia], ad1 = p.ad_off[ia + 1];
+    const VT* __restrict__ Tsrc = kRowLds ? trows + (size_t)r * nb : p.T + ia * nb;
+    VT* __restrict__ Wrow = p.W + ia * nb;
+    const VT* __restrict__ hrow = p.hii + ia * nb;
+    AT acc[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      const VT hv = hrow[col[j]];
+      const VT ts = Tsrc[col[j]];
+      acc[j] = p.accumulate ? (AT)Wrow[col[j]] : (AT)0;
+      acc[j] += (AT)hv * (AT)ts;
+    }
+    // alpha singles (task type 2, OneExcite)
+    for (uint32_t q = as0; q < as1; q++) {
+      const PairRec rc = p.as_rec[q];
+      const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
+#pragma unroll
+      for (int j = 0; j < C; j++) {
+        const VT tk = Tk[col[j]];
+        const double e = kCache ? p.Ea[(uint64_t)q * nb + col[j]]
+                                : alpha_single_elem(rc, A[r], cr[j].B, L, p.G);
+        acc[j] += (AT)e * (AT)tk;
+      }
+    }
+    // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
+    for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
+      VT tv[kBatch][C];
+      double vv[kBatch];
+      uint64_t mm[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        if (q0 + u < ad1) {
+          const PairRec rc = p.ad_rec[q0 + u];
+          const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
+#pragma unroll
+          for (int j = 0; j < C; j++) tv[u][j] = Tk[col[j]];
+          vv[u] = rc.val;
+          mm[u] = beta_mask_double(rc.code);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        if (q0 + u < ad1) {
+#pragma unroll
+          for (int j = 0; j < C; j++) {
+            const double e = flip_sign(vv[u], (uint32_t)__popcll(cr[j].B & mm[u]));
+            acc[j] += (AT)e * (AT)tv[u][j];
+          }
+        }
+      }
+    }
+    // beta singles (task type 1, OneExcite): rare, per column
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      const uint32_t s1 = ok[j] ? cr[j].s1 : cr[j].s0;
+      for (uint32_t q = cr[j].s0; q < s1; q++) {
+        const PairRec rc = p.bs_rec[q];
+        const double e = kCache ? p.Eb[ia * (uint64_t)p.Sb + q]
+                                : beta_single_elem(rc, A[r], cr[j].B, L, p.G);
+        acc[j] += (AT)e * (AT)Tsrc[rc.ket];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      if (ok[j]) Wrow[col[j]] = (VT)acc[j];
+  }
+}
+
+// Phase 2 of one 1024-column block: thread t (of the second half) owns the
+// columns col[j] at block-sorted positions blk + t + 512 j (the block's columns
+// in descending order of their beta-double count; columns loaded ahead),
+// reloads their partial sums and adds the beta doubles and the alpha-single x
+// beta-single terms of every row of the group.  Each wavefront takes 64
+// positions of nearly equal count, so its ELL slices carry little padding;
+// the slots of a slice sit at fixed strides from one base and are fetched
+// together with the partial sums (their addresses depend on nothing loaded),
+// once for all rows of the group.  Returns the thread's share of <T, W> when
+// `dot` is set.
+template <typename VT, bool kRowLds, int kRows, bool kPk>
+__device__ inline double sigma_phase2(const SigmaArgs<VT>& p, const VT* __restrict__ trows,
+                                      const uint64_t* A, uint64_t ia0, int nr, uint32_t blk,
+                                      uint32_t t, const EllPos* pc, bool dot) {
+  using AT = VT;
+  constexpr int C = kSigCols;
+  constexpr int BB = kBBatch<VT, kRows>;
+  const uint32_t nb = (uint32_t)p.nb;
+  const int L = p.L;
+  double tw = 0.0;
+  bool okp[C];
+  uint32_t col[C], i0[C], w[C];
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const uint32_t pos = blk + j * kHalf + t;
+    okp[j] = pos < nb;
+    col[j] = pc[j].col;
+    // the slice's offset and width are uniform over the wavefront (a position
+    // past the end is clamped: its slice is then the last one)
+    const uint32_t sl = __builtin_amdgcn_readfirstlane((okp[j] ? pos : nb - 1) / kEllSlice);
+    const uint32_t o0 = p.bd_ell_off[sl];
+    w[j] = (p.bd_ell_off[sl + 1] - o0) / kEllSlice;
+    i0[j] = o0 + pos % kEllSlice;
+  }
+  AT acc[kRows][C];
+#pragma unroll
+  for (int r = 0; r < kRows; r++)
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      acc[r][j] = (r < nr && okp[j]) ? (AT)p.W[(ia0 + r) * nb + col[j]] : (AT)0;
+  uint32_t wmax = 0;
+#pragma unroll
+  for (int j = 0; j < C; j++) wmax = w[j] > wmax ? w[j] : wmax;
+  // beta doubles (task type 1, TwoExcite): slot k of position j at
+  // i0[j] + 64 k for k below the slice width (past the position's own count
+  // the slice holds zero slots).  A batch issues all its loads together;
+  // slots past the width read the zero slot after the last slice, so every
+  // product is added without a condition (an exact zero for those).
+  const uint32_t zs = p.bd_ell_off[(nb + kEllSlice - 1) / kEllSlice];
+  for (uint32_t k0 = 0; k0 < wmax; k0 += BB) {
+    uint32_t kk[C][BB];
+    VT vv[C][BB];
+    uint64_t mm[C][BB];
+#pragma unroll
+    for (int j = 0; j < C; j++)
+#pragma unroll
+      for (int u = 0; u < BB; u++)
+        EllOf<VT>::template load<kPk>(p, k0 + u < w[j] ? i0[j] + (k0 + u) * kEllSlice : zs,
+                                      kk[j][u], vv[j][u], mm[j][u]);
+#pragma unroll
+    for (int r = 0; r < kRows; r++) {
+      if (r >= nr) continue;
+      const VT* __restrict__ Tsrc = kRowLds ? trows + (size_t)r * nb : p.T + (ia0 + r) * nb;
+#pragma unroll
+      for (int j = 0; j < C; j++) {
+        VT tt[BB];
+#pragma unroll
+        for (int u = 0; u < BB; u++) tt[u] = Tsrc[kk[j][u]];
+#pragma unroll
+        for (int u = 0; u < BB; u++) {
+          const VT e = flip_sign(vv[j][u], (uint32_t)__popcll(A[r] & mm[j][u]));
+          acc[r][j] += e * tt[u];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; r++) {
+    if (r >= nr) continue;
+    const uint64_t ia = ia0 + r;
+    const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      if (!okp[j]) continue;
+      // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by)):
+      // only rows with alpha singles (uniform) look at the column's singles
+      if (as1 > as0) {
+        const ColRec cb = p.b_rec[col[j]];
+        if (cb.s1 > cb.s0) {
+          const uint64_t D0 = spread32(A[r]) | (spread32(cb.B) << 1);
+          const uint64_t D1 = spread32(A[r] >> 32) | (spread32(cb.B >> 32) << 1);
+          for (uint32_t q = as0; q < as1; q++) {
+            const PairRec ra = p.as_rec[q];
+            const int xa = code_x0(ra.code), ya = code_y0(ra.code);
+            const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
+            // beta records kABatch at a time: their ket and integral loads are
+            // issued together, the sums are added in order (slots past the
+            // end repeat the last record and are discarded)
+            for (uint32_t k0 = cb.s0; k0 < cb.s1; k0 += kABatch) {
+              uint32_t code[kABatch];
+              VT tv[kABatch];
+#pragma unroll
+              for (int u = 0; u < kABatch; u++) {
+                const PairRec rb = p.bs_rec[k0 + u < cb.s1 ? k0 + u : cb.s1 - 1];
+                code[u] = rb.code;
+                tv[u] = Tk[rb.ket];
+              }
+#pragma unroll
+              for (int u = 0; u < kABatch; u++) {
+                const double e =
+                    ab_elem(D0, D1, xa, ya, code_x0(code[u]), code_y0(code[u]), L, p.Vab);
+                const AT nv = acc[r][j] + (AT)e * (AT)tv[u];
+                acc[r][j] = (k0 + u < cb.s1) ? nv : acc[r][j];
+              }
+            }
+          }
+        }
+      }
+      p.W[ia * nb + col[j]] = (VT)acc[r][j];
+      if (dot) {
+        const VT tc = kRowLds ? trows[(size_t)r * nb + col[j]] : p.T[ia * nb + col[j]];
+        tw += (double)tc * (double)(VT)acc[r][j];
+      }
+    }
+  }
+  return tw;
+}
+
 // One workgroup per kRows consecutive bra alpha rows (grid-stride over row
-// groups); the beta columns are taken kSigBlock at a time.  Each element is
-// accumulated in the reference order, in two phases per block of columns:
-//   1. thread t owns column blk + t of each row: diagonal, alpha singles,
-//      alpha doubles and beta singles; the partial sum is stored to W;
-//   2. after a barrier, thread t owns the column at block-sorted position
-//      blk + t (the block's columns in descending order of their beta-double
-//      count), reloads its partial sums and adds the beta doubles and the
-//      alpha-single x beta-single terms of every row of the group.
-// Phase 1 keeps every access to T, hii and W coalesced; phase 2 gives each
-// wavefront 64 columns of nearly equal beta-double count, so its ELL slice
-// carries little padding and its lanes finish together.  With kRows > 1 (the
-// rows fit LDS together) each beta-double slot is fetched once for all rows.
+// groups); the beta columns are taken in blocks of 1024, each in two phases
+// (phase 1 and phase 2 above).  The two halves of the workgroup work as a
+// pipeline: while the first eight wavefronts run phase 1 of block g (streams
+// from HBM), the other eight run phase 2 of block g - 1 (ELL slots, mostly from
+// L2, and gathers from the LDS rows); one barrier per block hands the partial
+// sums over through W.  Each half loads the per-column (per-position) records
+// of its next block before the barrier, so a phase starts with every address
+// it needs.  Each element is accumulated in the reference order: diagonal,
+// alpha singles, alpha doubles, beta singles (phase 1), beta doubles and
+// alpha-single x beta-single terms (phase 2).  The bra rows are staged in LDS
+// (when they fit) because every beta excitation of (ia, ib) lands in them.
 // Every pointer is a restrict-qualified kernel argument: W aliases no input,
 // so the workgroup-uniform alpha records are fetched with scalar loads.
-// Row groups bid, bid + nblk, ... of one sigma (the workgroup's share);
+// Row groups bid, bid + ngrp, ... of one sigma (the workgroup's share);
 // dot_part (optional) receives this workgroup's per-wavefront <T, W> sums at
 // [bid * 16 + wave].
-template <typename VT, typename AT, bool kRowLds, bool kCache, int kRows>
-__device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
-                                       const EllF32* __restrict__ bd_ell_f32_, uint32_t bid,
-                                       uint32_t ngrp, double* __restrict__ dot_part) {
-  static_assert(std::is_same<VT, AT>::value, "partial sums round-trip through W");
+template <typename VT, bool kRowLds, bool kCache, int kRows, bool kPk>
+__device__ inline void sigma_rows_body(const SigmaArgs<VT>& p, uint32_t bid, uint32_t ngrp,
+                                       double* __restrict__ dot_part) {
   static_assert(kRowLds || kRows == 1, "row groups need the rows in LDS");
+  static_assert(kSigCols * kHalf == kColBlock, "each half covers one block");
+  constexpr int C = kSigCols;
   extern __shared__ unsigned char lds_raw[];
   VT* trows = reinterpret_cast<VT*>(lds_raw);
   const uint32_t nb = (uint32_t)p.nb;
-  const int L = p.L;
-  const uint32_t nblk = (nb + kSigBlock - 1) / kSigBlock;
+  const uint32_t nblk = (nb + kColBlock - 1) / kColBlock;
+  const bool streamer = threadIdx.x < kHalf;  // uniform over each wavefront
+  const uint32_t t = streamer ? threadIdx.x : threadIdx.x - kHalf;
   double tw = 0.0;  // sum of T * W over this thread's outputs (when dot_part is set)
   for (uint64_t ia0 = (uint64_t)bid * kRows; ia0 < p.na; ia0 += (uint64_t)ngrp * kRows) {
     const int nr = (int)(p.na - ia0 < (uint64_t)kRows ? p.na - ia0 : kRows);  // uniform
@@ -323,155 +559,42 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
         if (r < nr) stage_row<VT>(trows + (size_t)r * nb, p.T + (ia0 + r) * nb, nb);
       __syncthreads();
     }
-    // per-row values are re-read where they are used (workgroup-uniform scalar
-    // loads) rather than held for all rows, which would not fit the registers
-    // with several rows per workgroup
-    for (uint32_t g = 0; g < nblk; g++) {
-      const uint32_t ib = g * kSigBlock + threadIdx.x;
-      // ---- phase 1: column ib of each row
-      if (ib < nb) {
-        const uint64_t B = p.b[ib];
-        const uint32_t bs0 = p.bs_off[ib], bs1 = p.bs_off[ib + 1];
-#pragma unroll 1
-        for (int r = 0; r < nr; r++) {
-          const uint64_t ia = ia0 + r;
-          const uint64_t A = p.a[ia];
-          const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
-          const uint32_t ad0 = p.ad_off[ia], ad1 = p.ad_off[ia + 1];
-          const VT* __restrict__ Tsrc = kRowLds ? trows + (size_t)r * nb : p.T + ia * nb;
-          VT* __restrict__ Wrow = p.W + ia * nb;
-          const VT hv = p.hii[ia * nb + ib];
-          const VT ts = Tsrc[ib];
-          AT acc = p.accumulate ? (AT)Wrow[ib] : (AT)0;
-          acc += (AT)hv * (AT)ts;
-          // alpha singles (task type 2, OneExcite): workgroup-uniform records
-          for (uint32_t q = as0; q < as1; q++) {
-            const PairRec rc = p.as_rec[q];
-            const VT tk = p.T[(uint64_t)rc.ket * nb + ib];
-            const double e = kCache ? p.Ea[(uint64_t)q * nb + ib]
-                                    : alpha_single_elem(rc, A, B, L, p.G);
-            acc += (AT)e * (AT)tk;
-          }
-          // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
-          for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
-            VT tv[kBatch];
-            double vv[kBatch];
-            uint64_t mm[kBatch];
+    uint64_t A[kRows];
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-              if (q0 + u < ad1) {
-                const PairRec rc = p.ad_rec[q0 + u];
-                tv[u] = p.T[(uint64_t)rc.ket * nb + ib];
-                vv[u] = rc.val;
-                mm[u] = beta_mask_double(rc.code);
-              }
-            }
+    for (int r = 0; r < kRows; r++) A[r] = r < nr ? p.a[ia0 + r] : 0;
+    // records of block 0 (columns for phase 1, block-sorted columns for phase 2)
+    ColRec cr[C];
+    EllPos pc[C];
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-              if (q0 + u < ad1) {
-                const double e = (__popcll(B & mm[u]) & 1) ? -vv[u] : vv[u];
-                acc += (AT)e * (AT)tv[u];
-              }
+    for (int j = 0; j < C; j++) {
+      const uint32_t i = j * kHalf + t < nb ? j * kHalf + t : nb - 1;
+      if (streamer) cr[j] = p.b_rec[i];
+      else pc[j] = p.bd_ell_pos[i];
+    }
+    for (uint32_t g = 0; g <= nblk; g++) {
+      if (streamer) {
+        if (g < nblk && g * kColBlock + t < nb) {
+          sigma_phase1<VT, kRowLds, kCache>(p, trows, A, ia0, nr, g * kColBlock, t, cr);
+          if (g + 1 < nblk) {
+#pragma unroll
+            for (int j = 0; j < C; j++) {
+              const uint32_t c = (g + 1) * kColBlock + j * kHalf + t;
+              cr[j] = p.b_rec[c < nb ? c : nb - 1];
             }
           }
-          // beta singles (task type 1, OneExcite): rare, per column
-          for (uint32_t q = bs0; q < bs1; q++) {
-            const PairRec rc = p.bs_rec[q];
-            const double e =
-                kCache ? p.Eb[ia * (uint64_t)p.Sb + q] : beta_single_elem(rc, A, B, L, p.G);
-            acc += (AT)e * (AT)Tsrc[rc.ket];
+        }
+      } else if (g > 0 && (g - 1) * kColBlock + t < nb) {
+        tw += sigma_phase2<VT, kRowLds, kRows, kPk>(p, trows, A, ia0, nr, (g - 1) * kColBlock,
+                                                    t, pc, dot_part != nullptr);
+        if (g < nblk) {
+#pragma unroll
+          for (int j = 0; j < C; j++) {
+            const uint32_t i = g * kColBlock + j * kHalf + t;
+            pc[j] = p.bd_ell_pos[i < nb ? i : nb - 1];
           }
-          Wrow[ib] = (VT)acc;
         }
       }
       __syncthreads();
-      // ---- phase 2: the column at block-sorted position ib, every row
-      if (ib < nb) {
-        const EllPos ps = p.bd_ell_pos[ib];
-        const uint32_t c = ps.col, cnt = ps.cnt;
-        AT acc[kRows];
-#pragma unroll
-        for (int r = 0; r < kRows; r++) acc[r] = r < nr ? (AT)p.W[(ia0 + r) * nb + c] : (AT)0;
-        // beta doubles (task type 1, TwoExcite), sliced-ELL: slot k of this
-        // position is at base + 64k; slots past the count re-read a valid
-        // record and their sum is discarded with a select (bit-identical to
-        // skipping it)
-        const uint32_t sl = __builtin_amdgcn_readfirstlane(ib / kEllSlice);
-        const size_t base = p.bd_ell_off[sl] + ib % kEllSlice;
-        for (uint32_t k0 = 0; k0 < cnt; k0 += kBBatch) {
-          uint32_t kk[kBBatch];
-          VT vv[kBBatch];
-          uint64_t mm[kBBatch];
-#pragma unroll
-          for (int u = 0; u < kBBatch; u++) {
-            const uint32_t k = k0 + u < cnt ? k0 + u : cnt - 1;
-            EllRead<VT>::load(p.bd_ell_ket, p.bd_ell_vm, bd_ell_f32_, base + (size_t)k * kEllSlice,
-                              kk[u], vv[u], mm[u]);
-          }
-#pragma unroll
-          for (int r = 0; r < kRows; r++) {
-            if (r >= nr) continue;
-            const VT* __restrict__ Tsrc =
-                kRowLds ? trows + (size_t)r * nb : p.T + (ia0 + r) * nb;
-            const uint64_t A = p.a[ia0 + r];
-            VT tt[kBBatch];
-#pragma unroll
-            for (int u = 0; u < kBBatch; u++) tt[u] = Tsrc[kk[u]];
-#pragma unroll
-            for (int u = 0; u < kBBatch; u++) {
-              const VT e = (__popcll(A & mm[u]) & 1) ? -vv[u] : vv[u];
-              const AT nv = acc[r] + (AT)e * (AT)tt[u];
-              acc[r] = (k0 + u < cnt) ? nv : acc[r];
-            }
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < kRows; r++) {
-          if (r >= nr) continue;
-          const uint64_t ia = ia0 + r;
-          // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by))
-          const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
-          if (as1 > as0) {
-            const uint32_t bs0 = p.bs_off[c], bs1 = p.bs_off[c + 1];
-            if (bs1 > bs0) {
-              const uint64_t A = p.a[ia];
-              const uint64_t B = p.b[c];
-              const uint64_t D0 = spread32(A) | (spread32(B) << 1);
-              const uint64_t D1 = spread32(A >> 32) | (spread32(B >> 32) << 1);
-              for (uint32_t q = as0; q < as1; q++) {
-                const PairRec ra = p.as_rec[q];
-                const int xa = code_x0(ra.code), ya = code_y0(ra.code);
-                const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
-                // beta records kABatch at a time: their ket and integral loads
-                // are issued together, the sums are added in order (slots
-                // past the end repeat the last record and are discarded)
-                for (uint32_t k0 = bs0; k0 < bs1; k0 += kABatch) {
-                  uint32_t code[kABatch];
-                  VT tv[kABatch];
-#pragma unroll
-                  for (int u = 0; u < kABatch; u++) {
-                    const PairRec rb = p.bs_rec[k0 + u < bs1 ? k0 + u : bs1 - 1];
-                    code[u] = rb.code;
-                    tv[u] = Tk[rb.ket];
-                  }
-#pragma unroll
-                  for (int u = 0; u < kABatch; u++) {
-                    const double e =
-                        ab_elem(D0, D1, xa, ya, code_x0(code[u]), code_y0(code[u]), L, p.Vab);
-                    const AT nv = acc[r] + (AT)e * (AT)tv[u];
-                    acc[r] = (k0 + u < bs1) ? nv : acc[r];
-                  }
-                }
-              }
-            }
-          }
-          p.W[ia * nb + c] = (VT)acc[r];
-          if (dot_part) {
-            const VT tc = kRowLds ? trows[(size_t)r * nb + c] : p.T[ia * nb + c];
-            tw += (double)tc * (double)(VT)acc[r];
-          }
-        }
-      }
     }
   }
   if (dot_part) {
@@ -480,31 +603,31 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p,
   }
 }
 
-template <typename VT, typename AT, bool kRowLds, bool kCache, int kRows>
+template <typename VT, bool kRowLds, bool kCache, int kRows, bool kPk>
 __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint64_t* __restrict__ a_, const uint64_t* __restrict__ b_, uint64_t na_, uint64_t nb_,
     int L_, const VT* __restrict__ hii_, const VT* __restrict__ T_, VT* __restrict__ W_,
     int accumulate_, const uint32_t* __restrict__ as_off_, const PairRec* __restrict__ as_rec_,
     const uint32_t* __restrict__ ad_off_, const PairRec* __restrict__ ad_rec_,
     const uint32_t* __restrict__ bs_off_, const PairRec* __restrict__ bs_rec_,
-    const uint32_t* __restrict__ bd_off_, const uint32_t* __restrict__ bd_ell_ket_,
-    const ValMask* __restrict__ bd_ell_vm_, const EllF32* __restrict__ bd_ell_f32_,
+    const uint32_t* __restrict__ bd_off_, const EllSlot64* __restrict__ bd_ell64_,
+    const EllSlot32* __restrict__ bd_ell32_, const uint32_t* __restrict__ bd_ell_ket_,
     const uint32_t* __restrict__ bd_ell_off_, const EllPos* __restrict__ bd_ell_pos_,
-    const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
+    const ColRec* __restrict__ b_rec_, const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
     const double* __restrict__ G_, const double* __restrict__ I2_,
     const double* __restrict__ Vab_, double* __restrict__ dot_part) {
-  const SigmaArgs<VT> p{a_,      b_,          na_,         nb_,         L_,          hii_,
-                        T_,      W_,          accumulate_, as_off_,     as_rec_,     ad_off_,
-                        ad_rec_, bs_off_,     bs_rec_,     bd_off_,     bd_ell_ket_, bd_ell_vm_,
-                        bd_ell_off_, bd_ell_pos_, Ea_,     Eb_,         Sb_,         G_,
-                        I2_,     Vab_};
-  sigma_rows_body<VT, AT, kRowLds, kCache, kRows>(p, bd_ell_f32_, blockIdx.x, gridDim.x,
-                                                  dot_part);
+  const SigmaArgs<VT> p{a_,       b_,          na_,         nb_,         L_,        hii_,
+                        T_,       W_,          accumulate_, as_off_,     as_rec_,   ad_off_,
+                        ad_rec_,  bs_off_,     bs_rec_,     bd_off_,     bd_ell64_, bd_ell32_,
+                        bd_ell_ket_, bd_ell_off_, bd_ell_pos_, b_rec_,  Ea_,       Eb_,
+                        Sb_,      G_,          I2_,         Vab_};
+  sigma_rows_body<VT, kRowLds, kCache, kRows, kPk>(p, blockIdx.x, gridDim.x, dot_part);
 }
 
 // Several sigmas in one launch (the serving engine's batch): job j owns
 // workgroups [blk0, blk0 + ngrp) and each of them one row group, so a job's
 // results, including its <T, W> partials, do not depend on the other jobs.
+// Jobs here have packed tables, single-excitation caches and rows in LDS.
 template <typename VT, int kRows>
 __global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* __restrict__ jobs,
                                                            int njobs) {
@@ -515,8 +638,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* _
     else hi = mid - 1;
   }
   const SigmaJob<VT>& J = jobs[lo];
-  sigma_rows_body<VT, VT, true, true, kRows>(J.p, J.ell_f32, blockIdx.x - J.blk0, J.ngrp,
-                                             J.dot_part);
+  sigma_rows_body<VT, true, true, kRows, true>(J.p, blockIdx.x - J.blk0, J.ngrp, J.dot_part);
 }
 
 // Sum of n partials in a fixed order: thread t adds partials t, t + 1024, ...
@@ -596,8 +718,9 @@ static SigmaArgs<double> sigma_args(const sbd_ctx* ctx, const double* hii, const
   const SpinTables& B = ctx->tb;
   return SigmaArgs<double>{ctx->d_a,  ctx->d_b,   ctx->na,    ctx->nb,   ctx->norb, hii,
                            T,         W,          0,          A.s_off,   A.s_rec,   A.d_off,
-                           A.d_rec,   B.s_off,    B.s_rec,    B.d_off,   B.ell_ket, B.ell_vm,
-                           B.ell_off, B.ell_pos,  ctx->d_Ea,  ctx->d_Eb, (uint32_t)B.n_s,
+                           A.d_rec,   B.s_off,    B.s_rec,    B.d_off,   B.ell64,   B.ell32,
+                           B.ell_ket, B.ell_off,  B.ell_pos,  B.col_rec, ctx->d_Ea,
+                           ctx->d_Eb, (uint32_t)B.n_s,
                            ctx->d_G,  ctx->d_eri, ctx->d_Vab};
 }
 
@@ -623,7 +746,7 @@ int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* 
     const bool lds = sizeof(double) * c->nb <= (size_t)c->lds_max;
     const int r = lds ? sigma_rows_for(c, sizeof(double)) : 1;
     const uint64_t groups = (c->na + r - 1) / r;
-    if (!cache || !lds || groups > kSigMaxGrid) direct.push_back(j);
+    if (!cache || !lds || !c->tb.ell_packed || groups > kSigMaxGrid) direct.push_back(j);
     else byrows[r == 4 ? 2 : r == 2 ? 1 : 0].push_back(j);
   }
   // jobs outside the batched form: their own launch (same row groups and sums)
@@ -644,7 +767,6 @@ int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* 
       const SigmaItem& it = items[sel[q]];
       SigmaJob<double>& J = jobs[q];
       J.p = sigma_args(it.ctx, it.hii, it.T, it.W);
-      J.ell_f32 = it.ctx->tb.ell_f32;
       J.ngrp = (uint32_t)((it.ctx->na + R - 1) / R);
       J.blk0 = blk;
       blk += J.ngrp;
@@ -725,12 +847,19 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
   const SpinTables& B = ctx->tb;
 #define SBD_SIGMA_ARGS                                                                          \
   ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,     \
-      A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell_ket, B.ell_vm, B.ell_f32, B.ell_off,   \
-      B.ell_pos, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,           \
+      A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell64, B.ell32, B.ell_ket, B.ell_off,      \
+      B.ell_pos, B.col_rec,                                                                     \
+      (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,                    \
       (const double*)ctx->d_G, (const double*)ctx->d_eri, (const double*)ctx->d_Vab, dpart
-#define SBD_LAUNCH(LDS, CACHE, ROWS)                                                            \
-  hipLaunchKernelGGL((k_sigma_rows<VT, VT, LDS, CACHE, ROWS>), dim3(grid), dim3(kSigBlock), shm, \
-                     st, SBD_SIGMA_ARGS)
+#define SBD_LAUNCH(LDS, CACHE, ROWS)                                                             \
+  do {                                                                                           \
+    if (B.ell_packed)                                                                            \
+      hipLaunchKernelGGL((k_sigma_rows<VT, LDS, CACHE, ROWS, true>), dim3(grid), dim3(kSigBlock), \
+                         shm, st, SBD_SIGMA_ARGS);                                               \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_sigma_rows<VT, LDS, CACHE, ROWS, false>), dim3(grid),                \
+                         dim3(kSigBlock), shm, st, SBD_SIGMA_ARGS);                              \
+  } while (0)
   // precomputed single-excitation elements, when both caches exist
   const bool cache = (A.n_s == 0 || ctx->d_Ea) && (B.n_s == 0 || ctx->d_Eb);
   if (lds && rows == 8) {
@@ -775,20 +904,34 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
   SBD_HIP_TRY(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                          ctx->device));
   const void* fns[] = {
-      (const void*)k_sigma_rows<double, double, true, true, 1>,
-      (const void*)k_sigma_rows<double, double, true, false, 1>,
-      (const void*)k_sigma_rows<double, double, true, true, 2>,
-      (const void*)k_sigma_rows<double, double, true, false, 2>,
-      (const void*)k_sigma_rows<double, double, true, true, 4>,
-      (const void*)k_sigma_rows<double, double, true, false, 4>,
-      (const void*)k_sigma_rows<float, float, true, true, 1>,
-      (const void*)k_sigma_rows<float, float, true, false, 1>,
-      (const void*)k_sigma_rows<float, float, true, true, 2>,
-      (const void*)k_sigma_rows<float, float, true, false, 2>,
-      (const void*)k_sigma_rows<float, float, true, true, 4>,
-      (const void*)k_sigma_rows<float, float, true, false, 4>,
-      (const void*)k_sigma_rows<float, float, true, true, 8>,
-      (const void*)k_sigma_rows<float, float, true, false, 8>,
+      (const void*)k_sigma_rows<double, true, true, 1, true>,
+      (const void*)k_sigma_rows<double, true, false, 1, true>,
+      (const void*)k_sigma_rows<double, true, true, 2, true>,
+      (const void*)k_sigma_rows<double, true, false, 2, true>,
+      (const void*)k_sigma_rows<double, true, true, 4, true>,
+      (const void*)k_sigma_rows<double, true, false, 4, true>,
+      (const void*)k_sigma_rows<float, true, true, 1, true>,
+      (const void*)k_sigma_rows<float, true, false, 1, true>,
+      (const void*)k_sigma_rows<float, true, true, 2, true>,
+      (const void*)k_sigma_rows<float, true, false, 2, true>,
+      (const void*)k_sigma_rows<float, true, true, 4, true>,
+      (const void*)k_sigma_rows<float, true, false, 4, true>,
+      (const void*)k_sigma_rows<float, true, true, 8, true>,
+      (const void*)k_sigma_rows<float, true, false, 8, true>,
+      (const void*)k_sigma_rows<double, true, true, 1, false>,
+      (const void*)k_sigma_rows<double, true, false, 1, false>,
+      (const void*)k_sigma_rows<double, true, true, 2, false>,
+      (const void*)k_sigma_rows<double, true, false, 2, false>,
+      (const void*)k_sigma_rows<double, true, true, 4, false>,
+      (const void*)k_sigma_rows<double, true, false, 4, false>,
+      (const void*)k_sigma_rows<float, true, true, 1, false>,
+      (const void*)k_sigma_rows<float, true, false, 1, false>,
+      (const void*)k_sigma_rows<float, true, true, 2, false>,
+      (const void*)k_sigma_rows<float, true, false, 2, false>,
+      (const void*)k_sigma_rows<float, true, true, 4, false>,
+      (const void*)k_sigma_rows<float, true, false, 4, false>,
+      (const void*)k_sigma_rows<float, true, true, 8, false>,
+      (const void*)k_sigma_rows<float, true, false, 8, false>,
       (const void*)k_sigma_batch<double, 1>,
       (const void*)k_sigma_batch<double, 2>,
       (const void*)k_sigma_batch<double, 4>};

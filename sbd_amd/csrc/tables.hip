@@ -107,13 +107,15 @@ __global__ __launch_bounds__(kTabBlock) void k_tab_fill(const uint64_t* __restri
   }
 }
 
-// Sliced-ELL copy of a doubles CSR table (one thread per block-sorted
-// position), with the other-spin sign mask of each record expanded from its
-// code.
-__global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off,
+// Sliced-ELL copy of a beta doubles CSR table (one thread per block-sorted
+// position), with the alpha-string sign mask of each record expanded from its
+// code: a beta excitation between spin orbitals 2u+1 and 2w+1 picks up the
+// parity of the alpha orbitals [u+1, w+1).
+__global__ void k_tab_ell(uint32_t n, const uint32_t* __restrict__ off,
                           const PairRec* __restrict__ rec, const uint32_t* __restrict__ ell_off,
-                          const EllPos* __restrict__ ell_pos, uint32_t* __restrict__ ell_ket,
-                          ValMask* __restrict__ ell_vm, EllF32* __restrict__ ell_f32) {
+                          const EllPos* __restrict__ ell_pos, int packed,
+                          EllSlot64* __restrict__ ell64, EllSlot32* __restrict__ ell32,
+                          uint32_t* __restrict__ ell_ket) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = i / kEllSlice, l = i % kEllSlice;
@@ -121,25 +123,22 @@ __global__ void k_tab_ell(uint32_t n, int spin, const uint32_t* __restrict__ off
   const uint32_t o0 = off[col], cnt = off[col + 1] - o0;
   const uint32_t w = (ell_off[s + 1] - ell_off[s]) / kEllSlice;
   for (uint32_t u = 0; u < w; u++) {
-    const uint32_t k = u < cnt ? u : (cnt ? cnt - 1 : 0);
-    uint32_t ket = 0;
+    uint32_t ket = col;
     double val = 0.0;
     uint64_t mask = 0;
-    if (cnt) {
-      const PairRec r = rec[o0 + k];
+    if (u < cnt) {
+      const PairRec r = rec[o0 + u];
+      const uint32_t c = r.code;
       ket = r.ket;
       val = r.val;
-      const uint32_t c = r.code;
-      // other spin's orbitals between the excitation's spin orbitals:
-      // alpha excitation -> beta [lo, hi); beta excitation -> alpha [lo+1, hi+1)
-      const int d = spin == 0 ? 0 : 1;
-      mask = bits_range(code_f(c, 0) + d, code_f(c, 1) + d) ^
-             bits_range(code_f(c, 2) + d, code_f(c, 3) + d);
+      mask = bits_range(code_f(c, 0) + 1, code_f(c, 1) + 1) ^
+             bits_range(code_f(c, 2) + 1, code_f(c, 3) + 1);
     }
+    const uint64_t mk = packed ? (mask | ((uint64_t)ket << kEllKetShift)) : mask;
     const size_t slot = ell_off[s] + (size_t)u * kEllSlice + l;
-    ell_ket[slot] = ket;
-    ell_vm[slot] = ValMask{val, mask};
-    ell_f32[slot] = EllF32{ket, (float)val, mask};
+    ell64[slot] = EllSlot64{val, mk};
+    ell32[slot] = EllSlot32{(float)val, (uint32_t)mk, (uint32_t)(mk >> 32)};
+    if (!packed) ell_ket[slot] = ket;
   }
 }
 
@@ -253,36 +252,60 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
   if (!ws.empty())
     SBD_HIP_TRY(ctx, hipMemcpy(T.with_s, ws.data(), sizeof(uint32_t) * ws.size(),
                                hipMemcpyHostToDevice));
-  // sliced-ELL copy of the doubles over block-sorted positions
-  std::vector<uint32_t> col(n);
-  for (uint32_t g0 = 0; g0 < n; g0 += kColBlock) {
-    const uint32_t g1 = std::min<uint64_t>(n, (uint64_t)g0 + kColBlock);
-    for (uint32_t i = g0; i < g1; i++) col[i] = i;
-    std::stable_sort(col.begin() + g0, col.begin() + g1,
-                     [&](uint32_t x, uint32_t y) { return hd[x] > hd[y]; });
+  // sliced-ELL copy of the beta doubles over block-sorted positions
+  if (spin == 1) {
+    std::vector<uint32_t> col(n);
+    for (uint32_t g0 = 0; g0 < n; g0 += kColBlock) {
+      const uint32_t g1 = std::min<uint64_t>(n, (uint64_t)g0 + kColBlock);
+      for (uint32_t i = g0; i < g1; i++) col[i] = i;
+      std::stable_sort(col.begin() + g0, col.begin() + g1,
+                       [&](uint32_t x, uint32_t y) { return hd[x] > hd[y]; });
+    }
+    std::vector<EllPos> pos(n);
+    for (uint32_t i = 0; i < n; i++) pos[i] = EllPos{col[i], hd[col[i]]};
+    SBD_HIP_TRY(ctx, hipMalloc(&T.ell_pos, sizeof(EllPos) * n));
+    SBD_HIP_TRY(ctx,
+                hipMemcpy(T.ell_pos, pos.data(), sizeof(EllPos) * n, hipMemcpyHostToDevice));
+    T.n_slices = (n + kEllSlice - 1) / kEllSlice;
+    std::vector<uint32_t> eoff(T.n_slices + 1, 0);
+    for (uint32_t s = 0; s < T.n_slices; s++) {
+      uint32_t w = 0;
+      for (uint32_t i = s * kEllSlice; i < std::min(n, (s + 1) * kEllSlice); i++)
+        w = std::max(w, hd[col[i]]);
+      eoff[s + 1] = eoff[s] + w * kEllSlice;
+    }
+    SBD_HIP_TRY(ctx, hipMalloc(&T.ell_off, sizeof(uint32_t) * (T.n_slices + 1)));
+    SBD_HIP_TRY(ctx, hipMemcpy(T.ell_off, eoff.data(), sizeof(uint32_t) * (T.n_slices + 1),
+                               hipMemcpyHostToDevice));
+    T.n_slots = (uint64_t)eoff[T.n_slices];
+    // packed slots: alpha strings below bit kEllKetShift, kets above it
+    T.ell_packed = ctx->norb <= kEllKetShift && n <= (1u << (64 - kEllKetShift));
+    // one zero slot (ket 0, value 0, mask 0) after the last slice: the kernel
+    // reads it in place of slots past a slice's width, adding an exact zero
+    const size_t ns = T.n_slots + 1;
+    SBD_HIP_TRY(ctx, hipMalloc(&T.ell64, sizeof(EllSlot64) * ns));
+    SBD_HIP_TRY(ctx, hipMalloc(&T.ell32, sizeof(EllSlot32) * ns));
+    if (!T.ell_packed) SBD_HIP_TRY(ctx, hipMalloc(&T.ell_ket, sizeof(uint32_t) * ns));
+    SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell64 + T.n_slots, 0, sizeof(EllSlot64), st));
+    SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell32 + T.n_slots, 0, sizeof(EllSlot32), st));
+    if (T.ell_ket) SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell_ket + T.n_slots, 0, sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_tab_ell, dim3((n + 255) / 256), dim3(256), 0, st, n, T.d_off, T.d_rec,
+                       T.ell_off, T.ell_pos, T.ell_packed ? 1 : 0, T.ell64, T.ell32, T.ell_ket);
+    SBD_HIP_TRY(ctx, hipGetLastError());
+    // per-string records: string and single-excitation range
+    std::vector<ColRec> cr(n);
+    {
+      std::vector<uint64_t> bs(n);
+      SBD_HIP_TRY(ctx, hipMemcpy(bs.data(), d_s, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+      uint32_t o = 0;
+      for (uint32_t i = 0; i < n; i++) {
+        cr[i] = ColRec{bs[i], o, o + hs[i]};
+        o += hs[i];
+      }
+    }
+    SBD_HIP_TRY(ctx, hipMalloc(&T.col_rec, sizeof(ColRec) * n));
+    SBD_HIP_TRY(ctx, hipMemcpy(T.col_rec, cr.data(), sizeof(ColRec) * n, hipMemcpyHostToDevice));
   }
-  std::vector<EllPos> pos(n);
-  for (uint32_t i = 0; i < n; i++) pos[i] = EllPos{col[i], hd[col[i]]};
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_pos, sizeof(EllPos) * n));
-  SBD_HIP_TRY(ctx, hipMemcpy(T.ell_pos, pos.data(), sizeof(EllPos) * n, hipMemcpyHostToDevice));
-  T.n_slices = (n + kEllSlice - 1) / kEllSlice;
-  std::vector<uint32_t> eoff(T.n_slices + 1, 0);
-  for (uint32_t s = 0; s < T.n_slices; s++) {
-    uint32_t w = 0;
-    for (uint32_t i = s * kEllSlice; i < std::min(n, (s + 1) * kEllSlice); i++)
-      w = std::max(w, hd[col[i]]);
-    eoff[s + 1] = eoff[s] + w * kEllSlice;
-  }
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_off, sizeof(uint32_t) * (T.n_slices + 1)));
-  SBD_HIP_TRY(ctx, hipMemcpy(T.ell_off, eoff.data(), sizeof(uint32_t) * (T.n_slices + 1),
-                             hipMemcpyHostToDevice));
-  const size_t nslot = (size_t)eoff[T.n_slices] + 1;
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_ket, sizeof(uint32_t) * nslot));
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_vm, sizeof(ValMask) * nslot));
-  SBD_HIP_TRY(ctx, hipMalloc(&T.ell_f32, sizeof(EllF32) * nslot));
-  hipLaunchKernelGGL(k_tab_ell, dim3((n + 255) / 256), dim3(256), 0, st, n, spin, T.d_off,
-                     T.d_rec, T.ell_off, T.ell_pos, T.ell_ket, T.ell_vm, T.ell_f32);
-  SBD_HIP_TRY(ctx, hipGetLastError());
   // bra string of every single record
   std::vector<uint32_t> sb(tot_s + 1, 0);
   for (uint32_t i = 0, q = 0; i < n; i++)
