@@ -299,12 +299,15 @@ struct EllOf<float> {
 
 // Phase 1 of one 1024-column block for the rows of a group: thread t (of the
 // first half) owns columns blk + t + 512 j (records cr[j], loaded ahead):
-// diagonal, alpha singles, alpha doubles and beta singles, in that order; the
-// partial sums are stored to W.  Every access to T, hii and W is a coalesced
-// stream along the columns; the alpha records are workgroup-uniform.  Columns
-// past the end are clamped to the last one (loads stay in bounds; nothing is
+// diagonal, alpha singles, alpha doubles and beta singles, in that order for
+// every row; the partial sums are stored to W.  The alpha records of the
+// group's rows are contiguous in the CSR, so they are walked as one list (each
+// record adds to its own row's sum, rows in order): the ket loads of all rows
+// are batched together.  Every access to T, hii and W is a coalesced stream
+// along the columns; the alpha records are workgroup-uniform.  Columns past
+// the end are clamped to the last one (loads stay in bounds; nothing is
 // stored).
-template <typename VT, bool kRowLds, bool kCache>
+template <typename VT, bool kRowLds, bool kCache, int kRows>
 __device__ inline void sigma_phase1(const SigmaArgs<VT>& p, const VT* __restrict__ trows,
                                     const uint64_t* A, uint64_t ia0, int nr, uint32_t blk,
                                     uint32_t t, const ColRec* cr) {
@@ -320,42 +323,60 @@ __device__ inline void sigma_phase1(const SigmaArgs<VT>& p, const VT* __restrict
     ok[j] = c < nb;
     col[j] = ok[j] ? c : nb - 1;
   }
-#pragma unroll 1
-  for (int r = 0; r < nr; r++) {
+  // record ranges: row r's records end at as_end[r] / ad_end[r]
+  uint32_t as_end[kRows], ad_end[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; r++) {
+    const uint64_t ia = ia0 + (r < nr ? r : nr - 1);
+    as_end[r] = p.as_off[ia + 1];
+    ad_end[r] = p.ad_off[ia + 1];
+  }
+  AT acc[kRows][C];
+#pragma unroll
+  for (int r = 0; r < kRows; r++) {
+    if (r >= nr) continue;
     const uint64_t ia = ia0 + r;
-    const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
-    const uint32_t ad0 = p.ad_off[ia], ad1 = p.ad_off[ia + 1];
     const VT* __restrict__ Tsrc = kRowLds ? trows + (size_t)r * nb : p.T + ia * nb;
-    VT* __restrict__ Wrow = p.W + ia * nb;
-    const VT* __restrict__ hrow = p.hii + ia * nb;
-    AT acc[C];
 #pragma unroll
     for (int j = 0; j < C; j++) {
-      const VT hv = hrow[col[j]];
+      const VT hv = p.hii[ia * nb + col[j]];
       const VT ts = Tsrc[col[j]];
-      acc[j] = p.accumulate ? (AT)Wrow[col[j]] : (AT)0;
-      acc[j] += (AT)hv * (AT)ts;
+      acc[r][j] = p.accumulate ? (AT)p.W[ia * nb + col[j]] : (AT)0;
+      acc[r][j] += (AT)hv * (AT)ts;
     }
-    // alpha singles (task type 2, OneExcite)
-    for (uint32_t q = as0; q < as1; q++) {
+  }
+  // alpha singles (task type 2, OneExcite), rows in order
+  {
+    const uint32_t q1 = as_end[nr - 1];
+    int r = 0;
+    for (uint32_t q = p.as_off[ia0]; q < q1; q++) {
+      while (q >= as_end[r]) r++;  // uniform
       const PairRec rc = p.as_rec[q];
       const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
 #pragma unroll
-      for (int j = 0; j < C; j++) {
-        const VT tk = Tk[col[j]];
-        const double e = kCache ? p.Ea[(uint64_t)q * nb + col[j]]
-                                : alpha_single_elem(rc, A[r], cr[j].B, L, p.G);
-        acc[j] += (AT)e * (AT)tk;
+      for (int rr = 0; rr < kRows; rr++) {
+        if (rr != r) continue;
+#pragma unroll
+        for (int j = 0; j < C; j++) {
+          const VT tk = Tk[col[j]];
+          const double e = kCache ? p.Ea[(uint64_t)q * nb + col[j]]
+                                  : alpha_single_elem(rc, A[rr], cr[j].B, L, p.G);
+          acc[rr][j] += (AT)e * (AT)tk;
+        }
       }
     }
-    // alpha doubles (task type 2, TwoExcite): ket loads batched ahead
-    for (uint32_t q0 = ad0; q0 < ad1; q0 += kBatch) {
+  }
+  // alpha doubles (task type 2, TwoExcite): ket loads batched ahead over the
+  // records of all rows
+  {
+    const uint32_t q1 = ad_end[nr - 1];
+    for (uint32_t q0 = p.ad_off[ia0]; q0 < q1; q0 += kBatch) {
       VT tv[kBatch][C];
       double vv[kBatch];
       uint64_t mm[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; u++) {
-        if (q0 + u < ad1) {
+        if (q0 + u < q1) {
           const PairRec rc = p.ad_rec[q0 + u];
           const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
 #pragma unroll
@@ -366,16 +387,28 @@ __device__ inline void sigma_phase1(const SigmaArgs<VT>& p, const VT* __restrict
       }
 #pragma unroll
       for (int u = 0; u < kBatch; u++) {
-        if (q0 + u < ad1) {
+        if (q0 + u < q1) {
+          int r = 0;
+#pragma unroll
+          for (int rr = 0; rr + 1 < kRows; rr++) r += q0 + u >= ad_end[rr] ? 1 : 0;
 #pragma unroll
           for (int j = 0; j < C; j++) {
             const double e = flip_sign(vv[u], (uint32_t)__popcll(cr[j].B & mm[u]));
-            acc[j] += (AT)e * (AT)tv[u][j];
+            const AT x = (AT)e * (AT)tv[u][j];
+#pragma unroll
+            for (int rr = 0; rr < kRows; rr++)
+              if (rr == r) acc[rr][j] += x;
           }
         }
       }
     }
-    // beta singles (task type 1, OneExcite): rare, per column
+  }
+  // beta singles (task type 1, OneExcite): rare, per column
+#pragma unroll
+  for (int r = 0; r < kRows; r++) {
+    if (r >= nr) continue;
+    const uint64_t ia = ia0 + r;
+    const VT* __restrict__ Tsrc = kRowLds ? trows + (size_t)r * nb : p.T + ia * nb;
 #pragma unroll
     for (int j = 0; j < C; j++) {
       const uint32_t s1 = ok[j] ? cr[j].s1 : cr[j].s0;
@@ -383,12 +416,12 @@ __device__ inline void sigma_phase1(const SigmaArgs<VT>& p, const VT* __restrict
         const PairRec rc = p.bs_rec[q];
         const double e = kCache ? p.Eb[ia * (uint64_t)p.Sb + q]
                                 : beta_single_elem(rc, A[r], cr[j].B, L, p.G);
-        acc[j] += (AT)e * (AT)Tsrc[rc.ket];
+        acc[r][j] += (AT)e * (AT)Tsrc[rc.ket];
       }
     }
 #pragma unroll
     for (int j = 0; j < C; j++)
-      if (ok[j]) Wrow[col[j]] = (VT)acc[j];
+      if (ok[j]) p.W[ia * nb + col[j]] = (VT)acc[r][j];
   }
 }
 
@@ -574,7 +607,7 @@ __device__ inline void sigma_rows_body(const SigmaArgs<VT>& p, uint32_t bid, uin
     for (uint32_t g = 0; g <= nblk; g++) {
       if (streamer) {
         if (g < nblk && g * kColBlock + t < nb) {
-          sigma_phase1<VT, kRowLds, kCache>(p, trows, A, ia0, nr, g * kColBlock, t, cr);
+          sigma_phase1<VT, kRowLds, kCache, kRows>(p, trows, A, ia0, nr, g * kColBlock, t, cr);
           if (g + 1 < nblk) {
 #pragma unroll
             for (int j = 0; j < C; j++) {

@@ -157,9 +157,13 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
   double ot[MAXK], oh[MAXK];
 #pragma unroll
   for (int j = 0; j < MAXK; j++) ot[j] = oh[j] = 0.0;
-  // all 2k chunks are held until t is known: above 8 vectors FP32 takes
-  // 8-byte chunks so that the registers leave room for a few waves per SIMD
-  auto body = [&]<int W>(uint64_t i) {
+  // FP64: all 2k chunks are held until t is known, so each is read once.
+  // FP32: w and r are accumulated as the chunks arrive and, once t is known,
+  // the 2k chunks are read again (from L2: this thread has just read them,
+  // and FP32 chunks are small enough to stay there) for the overlaps, so no
+  // chunk is held across the division and the registers leave room for
+  // several waves per SIMD.  Both give the same sums in the same order.
+  auto hold = [&]<int W>(uint64_t i) {
     double w[W], r[W];
     Chunk<VT, W> cv[MAXK], hv[MAXK];
 #pragma unroll
@@ -209,8 +213,63 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
       }
     }
   };
-  constexpr int P = (sizeof(VT) == 4 && MAXK > 8) ? 2 : chunk_of<VT>();
-  for_chunks<VT, decltype(body)&, P>(g, n, packed, body);
+  auto reread = [&]<int W>(uint64_t i) {
+    double w[W], r[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) w[e] = r[e] = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXK; j++) {
+      if (j < k) {
+        const Chunk<VT, W> cv = ldc<W>(C.p[j], i);
+        const Chunk<VT, W> hv = ldc<W>(HC.p[j], i);
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+          if (j == 0) {
+            w[e] = (double)cv.v[e] * u.c[0];
+            r[e] = (double)hv.v[e] * (-1.0 * u.c[0]);
+          } else {
+            w[e] += (double)cv.v[e] * u.c[j];
+            r[e] += (double)hv.v[e] * (-1.0 * u.c[j]);
+          }
+        }
+      }
+    }
+    const Chunk<VT, W> dv = ldc<W>(dii, i);
+    Chunk<VT, W> ts;
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      r[e] += E0 * w[e];
+      const double den = E0 - (double)dv.v[e];
+      const double tv = fabs(den) > eps_reg ? r[e] / den : r[e] / (den - eps_reg);
+      ts.v[e] = (VT)tv;
+    }
+    stc<W>(t, i, ts);
+    double tv[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      tv[e] = (double)ts.v[e];
+      rr.add(r[e] * r[e]);
+      ww.add(w[e] * w[e]);
+      tt.add(tv[e] * tv[e]);
+    }
+    // the second reads must not be moved above the division
+    uint64_t i2 = i;
+    asm volatile("" : "+v"(i2));
+#pragma unroll
+    for (int j = 0; j < MAXK; j++) {
+      if (j < k) {
+        const Chunk<VT, W> cv = ldc<W>(C.p[j], i2);
+        const Chunk<VT, W> hv = ldc<W>(HC.p[j], i2);
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+          ot[j] += (double)cv.v[e] * tv[e];
+          oh[j] += (double)hv.v[e] * tv[e];
+        }
+      }
+    }
+  };
+  if constexpr (sizeof(VT) == 8) for_chunks<VT>(g, n, packed, hold);
+  else for_chunks<VT>(g, n, packed, reread);
   double v[2 * MAXK + 3];
   v[0] = rr.s;
   v[1] = ww.s;
