@@ -21,9 +21,16 @@
  *   sbd::Davidson        chemistry/tpb/davidson.h:707-970,
  *                        chemistry/basic/davidson_thrust.h:97-104
  *                                                        sbd_davidson
- *   sbd::tpb::diag       chemistry/tpb/sbdiag.h:151-526 (method 0)
+ *   sbd::Lanczos         chemistry/tpb/lanczos.h:162-327 sbd_lanczos
+ *   sbd::tpb::diag       chemistry/tpb/sbdiag.h:151-526 (methods 0-3)
  *                                                        sbd_diag
  *   sbd::InnerProduct    framework/dm_vector.h:23-51     sbd_inner
+ *   sbd::DiagonalAdet/BdetReducedDensityMatrix chemistry/tpb/rdmat.h:11-58,
+ *                        136-182                         sbd_string_weights
+ *   sbd::OccupationDensity chemistry/tpb/occupation.h:10-95
+ *                                                        sbd_occupation
+ *   sbd::CarryOverAdet / CarryOverBdet chemistry/tpb/rdmat.h:60-134 (+ beta)
+ *                                                        sbd_carryover
  *
  * Conventions
  *   - Strings are packed one uint64 per alpha/beta string (orbital p -> bit p);
@@ -65,7 +72,8 @@ typedef struct sbd_ctx sbd_ctx;
  * the fields this library adds (precision, verbose).  sbd_options_default
  * fills the reference's defaults. */
 typedef struct {
-  int method;        /* 0 = Davidson (the only method here)                  */
+  int method;        /* 0 = Davidson, 2 = Lanczos; 1 and 3 (the reference's
+                        stored-H forms of 0 and 2) run as 0 and 2             */
   int max_it;        /* restarts; default 1                                    */
   int max_nb;        /* subspace size; default 10                              */
   double eps;        /* residual-norm tolerance; default 1e-4                  */
@@ -165,8 +173,20 @@ int sbd_inner(sbd_ctx* ctx, const double* x, const double* y, uint64_t n, double
 int sbd_davidson(sbd_ctx* ctx, const double* hii, double* W, const sbd_options* opt,
                  double* energy, double* hist, sbd_report* rep);
 
-/* sbd::tpb::diag, method 0, one rank (tpb/sbdiag.h:151-526): tables, hii,
- * initial vector (opt->init), Davidson, then E = <W, H W> with a fresh sigma.
+/* Restarted Lanczos on device vectors (tpb/lanczos.h:162-327): each of
+ * max_it passes runs the three-term recurrence from C[0] = W for up to max_nb
+ * steps with no reorthogonalisation, stops when E0 changes by less than eps
+ * between steps, and sets W = sum U[ib, 0] C[ib] (not renormalised, as the
+ * reference).  *energy is the last E0; hist (optional, host, 4*max_it*max_nb
+ * doubles) receives (E0, beta, seconds, 64) per step (beta 0 when the step
+ * ended before computing it).  rep->converged = 1 when the E0 test stopped it,
+ * rep->residual = the last |E0 change|. */
+int sbd_lanczos(sbd_ctx* ctx, const double* hii, double* W, const sbd_options* opt,
+                double* energy, double* hist, sbd_report* rep);
+
+/* sbd::tpb::diag, one rank (tpb/sbdiag.h:151-526): tables, hii, initial
+ * vector (opt->init), Davidson (method 0/1) or Lanczos (method 2/3), then
+ * E = <W, H W> with a fresh sigma.
  * W_out (optional, host, N doubles) receives the final vector.  hist
  * (optional, host, 4*hist_cap doubles) receives per Davidson step
  * (E0, |R|, seconds since the solver started, 32 or 64 = vector precision). */
@@ -188,6 +208,28 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
 int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* energies,
                    double* residuals, double* W_out, double* hist, int hist_cap,
                    sbd_report* rep);
+
+/* Measurements on a device vector W (N doubles, the layout above), as
+ * sbd::tpb::diag makes them after the solve (tpb/sbdiag.h:666-777).
+ *
+ * String weights (DiagonalAdet/BdetReducedDensityMatrix, tpb/rdmat.h:11-58,
+ * 136-182): wa[ia] = sum_ib W^2, wb[ib] = sum_ia W^2 (host arrays of na and nb
+ * doubles; fixed-order compensated sums). */
+int sbd_string_weights(sbd_ctx* ctx, const double* W, double* wa, double* wb, void* stream);
+/* Occupation density (OccupationDensity, tpb/occupation.h:10-95, for every
+ * orbital): density[2o] = alpha and density[2o+1] = beta occupation of
+ * orbital o (host array of 2 norb doubles), the reference's `density` output. */
+int sbd_occupation(sbd_ctx* ctx, const double* W, double* density, void* stream);
+/* Carry-over strings (CarryOverAdet / CarryOverBdet, tpb/rdmat.h:60-134; the
+ * selection of tpb/sbdiag.h:745-772): strings in descending order of weight
+ * (ties by index).  ratio == 0: keep up to and including the first string
+ * whose cumulative weight exceeds 1 - threshold (all strings if none does);
+ * ratio > 0: keep (size_t)(ratio * count) strings.  a_out / b_out (host,
+ * capacity na / nb) receive the kept strings in that order, *na_out / *nb_out
+ * their numbers, truncated[2] (optional) 1 - the kept weight per spin. */
+int sbd_carryover(sbd_ctx* ctx, const double* W, double threshold, double ratio,
+                  uint64_t* a_out, size_t* na_out, uint64_t* b_out, size_t* nb_out,
+                  double* truncated, void* stream);
 
 /* Device bytes sbd_diag would allocate for these options (vectors + tables +
  * integrals), for admission control. */

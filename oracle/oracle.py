@@ -54,6 +54,11 @@ def lib():
         L.orc_syev.argtypes = [C.c_int, dp, C.c_int, dp]
         L.orc_davidson.restype = C.c_int
         L.orc_davidson.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, dp]
+        L.orc_lanczos.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, dp]
+        L.orc_weights.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_occupation.argtypes = [C.c_void_p, dp, dp]
+        L.orc_carryover.restype = C.c_long
+        L.orc_carryover.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, lp, dp]
         L.orc_hdiag_rows.argtypes = [C.c_void_p, dp, C.c_long, C.c_long]
         L.orc_sigma_rows.argtypes = [C.c_void_p, dp, dp, dp, C.c_long, C.c_long]
         L.orc_det_from_ab.argtypes = [up, up, C.c_long, C.c_int, up]
@@ -235,6 +240,38 @@ class System:
         steps = lib().orc_davidson(self._h, _p(hii, C.c_double), _p(W, C.c_double), max_it, nb,
                                    eps, _p(hist, C.c_double))
         return W, hist[:2 * steps].reshape(steps, 2)
+
+    def lanczos(self, hii, W, max_it, nb, eps):
+        """orc_lanczos (lanczos.h:162-327): returns W and the per-step history
+        rows (E0, alpha, beta)."""
+        W = np.ascontiguousarray(W, dtype=np.float64).copy()
+        hist = np.zeros(3 * max_it * nb)
+        steps = lib().orc_lanczos(self._h, _p(hii, C.c_double), _p(W, C.c_double), max_it, nb,
+                                  eps, _p(hist, C.c_double))
+        return W, hist[:3 * steps].reshape(steps, 3)
+
+    def weights(self, W):
+        """rdmat.h:11-58, 136-182: (alpha string weights, beta string weights)."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        Da, Db = np.zeros(self.na), np.zeros(self.nb)
+        lib().orc_weights(self._h, _p(W, C.c_double), _p(Da, C.c_double), _p(Db, C.c_double))
+        return Da, Db
+
+    def occupation(self, W):
+        """occupation.h:10-95: density[2o] alpha, density[2o+1] beta."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        out = np.zeros(2 * self.L)
+        lib().orc_occupation(self._h, _p(W, C.c_double), _p(out, C.c_double))
+        return out
+
+    def carryover(self, W, spin, threshold, ratio=0.0):
+        """rdmat.h:60-134: indices of the kept strings and the truncated weight."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        out = np.zeros(self.nb if spin else self.na, dtype=np.int64)
+        tr = C.c_double(0.0)
+        m = lib().orc_carryover(self._h, _p(W, C.c_double), spin, threshold, ratio,
+                                out.ctypes.data_as(C.POINTER(C.c_long)), C.byref(tr))
+        return out[:m], tr.value
 
     def energy(self, hii, W):
         """sbdiag.h:488-526: E = <W, H W> with a fresh sigma."""

@@ -2,7 +2,8 @@
  * sbd_oracle.c -- TEST INFRASTRUCTURE ONLY.
  *
  * CPU restatement of the reference's tensor-product-basis (TPB) path: excitation
- * tables, diagonal, sigma (H*v) and single-root Davidson.  It is the parity
+ * tables, diagonal, sigma (H*v), single-root Davidson and Lanczos, and the
+ * measurements on a solved vector (string weights, occupations, carry-over).  It is the parity
  * checker for the HIP library and the timed CPU baseline; it is never part of
  * the product path.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline leg may load it.
@@ -534,6 +535,177 @@ int orc_davidson(const orc_sys *S, const double *hii, double *W, int max_it, int
   free(U);
   free(E);
   return steps;
+}
+
+/*
+ * Lanczos, chemistry/tpb/lanczos.h:162-327 (the operator form that diag calls
+ * for method 2, sbdiag.h:461-468), one rank.  Each iteration restarts the
+ * three-term recurrence from C[0] = W with no reorthogonalisation:
+ *   HC = 0; for ib: HC += H C[ib]; A[ib,ib] = <C[ib], HC>; E = eig(A[0:n,0:n]);
+ *   stop if |E[0] - E_old| < eps; HC -= A[ib,ib] C[ib]; C[ib+1] = HC;
+ *   A[ib,ib+1] = A[ib+1,ib] = |C[ib+1]| (then normalised); break if that is
+ *   below eps; HC = -A[ib,ib+1] C[ib].
+ * After the steps, W = sum_ib U[ib,0] C[ib] (not renormalised).  hist
+ * (optional, size max_it*nb*3) receives (E0, alpha, beta) per step, beta = 0
+ * when the step ended before computing it.  Returns the number of steps.
+ */
+int orc_lanczos(const orc_sys *S, const double *hii, double *W, int max_it, int nb, double eps,
+                double *hist) {
+  long N = S->na * S->nb;
+  double **C = (double **)malloc(nb * sizeof(double *));
+  for (int k = 0; k < nb; k++) C[k] = (double *)malloc(N * 8);
+  double *HC = (double *)malloc(N * 8);
+  double *A = (double *)calloc(nb * nb, 8), *U = (double *)calloc(nb * nb, 8);
+  double *E = (double *)malloc(nb * 8);
+  int steps = 0;
+  for (int it = 0; it < max_it; it++) {
+    int n = 0, stop = 0;
+    double E_old = 1.0e+8;
+    memcpy(C[0], W, N * 8);
+    memset(HC, 0, N * 8);
+    for (int ib = 0; ib < nb; ib++) {
+      n++;
+      int ii = ib + nb * ib, ij = ib + nb * (ib + 1), ji = ib + 1 + nb * ib;
+      orc_sigma(S, hii, C[ib], HC);
+      double Aii = orc_inner(C[ib], HC, N);
+      A[ii] = Aii;
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) U[i + nb * j] = A[i + nb * j];
+      orc_syev(n, U, nb, E);
+      if (hist) {
+        hist[3 * steps] = E[0];
+        hist[3 * steps + 1] = Aii;
+        hist[3 * steps + 2] = 0.0;
+      }
+      steps++;
+      if (fabs(E[0] - E_old) < eps) {
+        stop = 1;
+        break;
+      }
+      E_old = E[0];
+      if (ib + 1 == nb) break;
+#pragma omp parallel for
+      for (long is = 0; is < N; is++) HC[is] -= Aii * C[ib][is];
+      memcpy(C[ib + 1], HC, N * 8);
+      A[ij] = orc_normalize(C[ib + 1], N);
+      A[ji] = A[ij];
+      if (hist) hist[3 * (steps - 1) + 2] = A[ij];
+      if (fabs(A[ij]) < eps) break;
+#pragma omp parallel for
+      for (long is = 0; is < N; is++) HC[is] = -A[ij] * C[ib][is];
+    }
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) U[i + nb * j] = A[i + nb * j];
+    orc_syev(n, U, nb, E);
+    for (int ib = 0; ib < n; ib++) {
+      if (ib == 0) {
+#pragma omp parallel for
+        for (long is = 0; is < N; is++) W[is] = U[0] * C[ib][is];
+      } else {
+#pragma omp parallel for
+        for (long is = 0; is < N; is++) W[is] += U[ib] * C[ib][is];
+      }
+    }
+    if (stop) break;
+  }
+  for (int k = 0; k < nb; k++) free(C[k]);
+  free(C);
+  free(HC);
+  free(A);
+  free(U);
+  free(E);
+  return steps;
+}
+
+/*
+ * DiagonalAdetReducedDensityMatrix / DiagonalBdetReducedDensityMatrix,
+ * chemistry/tpb/rdmat.h:11-58, 136-182 (one rank): Da[ia] = sum_ib W^2 in ib
+ * order, Db[ib] = sum_ia W^2 in ia order.
+ */
+void orc_weights(const orc_sys *S, const double *W, double *Da, double *Db) {
+  for (long ia = 0; ia < S->na; ia++) {
+    double d = 0.0;
+    for (long ib = 0; ib < S->nb; ib++) d += W[ia * S->nb + ib] * W[ia * S->nb + ib];
+    Da[ia] = d;
+  }
+  for (long ib = 0; ib < S->nb; ib++) {
+    double d = 0.0;
+    for (long ia = 0; ia < S->na; ia++) d += W[ia * S->nb + ib] * W[ia * S->nb + ib];
+    Db[ib] = d;
+  }
+}
+
+/*
+ * OccupationDensity, chemistry/tpb/occupation.h:10-95 (one rank, every
+ * orbital): res[2o] = sum over determinants of W^2 if orbital o is occupied in
+ * the alpha string, res[2o+1] the same for beta (sbdiag.h:666-693 stores it as
+ * density[2o], density[2o+1]).
+ */
+void orc_occupation(const orc_sys *S, const double *W, double *res) {
+  const int L = S->L;
+  for (int o = 0; o < 2 * L; o++) res[o] = 0.0;
+  for (long ia = 0; ia < S->na; ia++)
+    for (long ib = 0; ib < S->nb; ib++) {
+      const double w2 = W[ia * S->nb + ib] * W[ia * S->nb + ib];
+      for (int o = 0; o < L; o++) {
+        if ((S->a[ia] >> o) & 1ULL) res[2 * o] += w2;
+        if ((S->b[ib] >> o) & 1ULL) res[2 * o + 1] += w2;
+      }
+    }
+}
+
+/*
+ * CarryOverAdet / CarryOverBdet, chemistry/tpb/rdmat.h:60-134 (spin 0 alpha, 1
+ * beta): strings sorted by descending weight (ties by index: the reference's
+ * std::sort leaves their order unspecified).  ratio == 0: keep up to and
+ * including the first string whose cumulative weight exceeds 1 - threshold
+ * (std::upper_bound + 1, clamped to all strings); else keep
+ * (size_t)(ratio * count) strings (sbdiag.h:745-772).  out receives the kept
+ * string indices; returns their number; *truncated = 1 - their weight.
+ */
+static const double *g_sort_w;
+static int cmp_desc(const void *x, const void *y) {
+  const long i = *(const long *)x, j = *(const long *)y;
+  if (g_sort_w[i] > g_sort_w[j]) return -1;
+  if (g_sort_w[i] < g_sort_w[j]) return 1;
+  return i < j ? -1 : (i > j ? 1 : 0);
+}
+long orc_carryover(const orc_sys *S, const double *W, int spin, double threshold, double ratio,
+                   long *out, double *truncated) {
+  double *Da = (double *)malloc(S->na * 8), *Db = (double *)malloc(S->nb * 8);
+  orc_weights(S, W, Da, Db);
+  const double *D = spin ? Db : Da;
+  const long n = spin ? S->nb : S->na;
+  long *order = (long *)malloc(n * sizeof(long));
+  for (long i = 0; i < n; i++) order[i] = i;
+  g_sort_w = D;
+  qsort(order, n, sizeof(long), cmp_desc);
+  long m;
+  if (ratio == 0.0) {
+    double sum = 0.0, target = 1.0 - threshold;
+    long first = n;
+    for (long i = 0; i < n; i++) {
+      sum += D[order[i]];
+      if (sum > target) {
+        first = i;
+        break;
+      }
+    }
+    m = first + 1 < n ? first + 1 : n;
+  } else {
+    m = (long)(ratio * (double)n);
+    if (m > n) m = n;
+  }
+  double kept = 0.0;
+  for (long i = 0; i < m; i++) {
+    out[i] = order[i];
+    kept += D[order[i]];
+  }
+  if (truncated) *truncated = 1.0 - kept;
+  free(order);
+  free(Da);
+  free(Db);
+  return m;
 }
 
 /* Rows [ia0, ia1) of hdiag and sigma (same arithmetic as orc_hdiag /

@@ -32,6 +32,7 @@ EXPORTS = [
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
     "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
+    "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_carryover",
     "sbd_diag", "sbd_diag_roots", "sbd_diag_footprint", "sbd_server_create", "sbd_server_submit",
     "sbd_server_wait", "sbd_server_poll", "sbd_server_now", "sbd_server_stats",
     "sbd_server_destroy",
@@ -109,6 +110,10 @@ def lib():
     L.sbd_sigma_f32.argtypes = [vp, vp, vp, vp, C.c_int, vp]
     L.sbd_inner.argtypes = [vp, vp, vp, C.c_uint64, dp, vp]
     L.sbd_davidson.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
+    L.sbd_lanczos.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
+    L.sbd_string_weights.argtypes = [vp, vp, dp, dp, vp]
+    L.sbd_occupation.argtypes = [vp, vp, dp, vp]
+    L.sbd_carryover.argtypes = [vp, vp, C.c_double, C.c_double, up, szp, up, szp, dp, vp]
     L.sbd_diag.argtypes = [vp, C.POINTER(Options), dp, dp, dp, C.c_int, C.POINTER(Report)]
     L.sbd_diag_footprint.argtypes = [vp, C.POINTER(Options), up]
     L.sbd_diag_roots.argtypes = [vp, C.POINTER(Options), C.c_int, dp, dp, dp, dp, C.c_int,
@@ -153,6 +158,10 @@ class SBD:
     init: int = 0
     seed: int = 1729
     bit_length: int = 20
+    do_rdm: int = 0              # 0: occupation density only (full RDMs not built)
+    carryover_type: int = 0      # 0 none, 1 dominant strings
+    ratio: float = 0.0
+    threshold: float = 0.01
     precision: int = 0           # added: 1 = FP32 vectors + FP64 correction
     fp32_eps: float = 1.0e-4     # added: FP32 -> FP64 hand-over residual
     verbose: int = 0
@@ -337,9 +346,26 @@ class MultTPB:
             out = out + (hist[:4 * rep.steps].reshape(-1, 4),)
         return out
 
+    def lanczos(self, hii, W, sbd: SBD, history=False):
+        """Restarted Lanczos (sbd::Lanczos, lanczos.h:162-327) on device vectors;
+        W (device, float64) is updated in place to sum U[ib, 0] C[ib].  History
+        rows are (E0, beta, seconds, 64)."""
+        o = sbd.options()
+        E = C.c_double(0.0)
+        rep = Report()
+        hist = np.zeros(4 * sbd.max_it * sbd.max_nb)
+        _check(lib().sbd_lanczos(self._ctx, _dptr(hii), _dptr(W), C.byref(o), C.byref(E),
+                                 _ptr(hist, C.c_double), C.byref(rep)), self._ctx)
+        out = (E.value, rep.as_dict())
+        if history:
+            out = out + (hist[:4 * rep.steps].reshape(-1, 4),)
+        return out
+
     def diag(self, sbd: SBD, want_vector=False, history=False):
-        """Returns (energy, W or None, report[, history]); history rows are
-        (E0, |R|, seconds since solver start, vector precision 32/64)."""
+        """sbd::tpb::diag: method 0/1 Davidson, 2/3 Lanczos.  Returns (energy,
+        W or None, report[, history]); Davidson history rows are (E0, |R|,
+        seconds since solver start, vector precision 32/64), Lanczos rows
+        (E0, beta, seconds, 64)."""
         o = sbd.options()
         E = C.c_double(0.0)
         rep = Report()
@@ -377,6 +403,31 @@ class MultTPB:
             out.append(h[: int(np.count_nonzero(h[:, 3]))])
         return tuple(out)
 
+    def string_weights(self, W):
+        """(alpha, beta) string weights of device vector W (rdmat.h:11-58, 136-182)."""
+        wa, wb = np.zeros(self.na), np.zeros(self.nb)
+        _check(lib().sbd_string_weights(self._ctx, _dptr(W), _ptr(wa, C.c_double),
+                                        _ptr(wb, C.c_double), _stream()), self._ctx)
+        return wa, wb
+
+    def occupation(self, W):
+        """OccupationDensity (occupation.h:10-95): density[2o] alpha, [2o+1] beta."""
+        out = np.zeros(2 * self.norb)
+        _check(lib().sbd_occupation(self._ctx, _dptr(W), _ptr(out, C.c_double), _stream()),
+               self._ctx)
+        return out
+
+    def carryover(self, W, threshold=0.01, ratio=0.0):
+        """CarryOverAdet/Bdet (rdmat.h:60-134): (alpha strings, beta strings,
+        truncated weights) in descending order of weight."""
+        ao, bo = np.zeros(self.na, dtype=np.uint64), np.zeros(self.nb, dtype=np.uint64)
+        na, nb = C.c_size_t(0), C.c_size_t(0)
+        tr = np.zeros(2)
+        _check(lib().sbd_carryover(self._ctx, _dptr(W), float(threshold), float(ratio),
+                                   _ptr(ao, C.c_uint64), C.byref(na), _ptr(bo, C.c_uint64),
+                                   C.byref(nb), _ptr(tr, C.c_double), _stream()), self._ctx)
+        return ao[:na.value].copy(), bo[:nb.value].copy(), tr
+
     def footprint(self, sbd: SBD):
         o = sbd.options()
         out = C.c_uint64(0)
@@ -384,16 +435,43 @@ class MultTPB:
         return out.value
 
 
+@dataclass
+class DiagResult:
+    """Outputs of sbd::tpb::diag (chemistry/tpb/sbdiag.h:151-163): energy,
+    occupation density (density[2o] alpha, [2o+1] beta), carry-over strings
+    (empty unless carryover_type is 1) and the final vector."""
+    energy: float
+    density: np.ndarray
+    co_adet: np.ndarray
+    co_bdet: np.ndarray
+    W: np.ndarray
+
+
 def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
-    """sbd::tpb::diag (method 0, one rank): returns (energy, W)."""
+    """sbd::tpb::diag (one rank; method 0/1 Davidson, 2/3 Lanczos), followed by
+    the occupation density and, with carryover_type 1, the carry-over strings
+    (sbdiag.h:666-777).  Returns a DiagResult."""
+    import torch
     if loadname or savename:
         raise SbdError("wavefunction load/save is not supported by this build")
+    if sbd_data.do_rdm != 0:
+        raise SbdError("do_rdm != 0 (full one- and two-particle RDMs) is not supported by this "
+                       "build; do_rdm = 0 gives the occupation density")
+    if sbd_data.carryover_type not in (0, 1):
+        raise SbdError("carryover_type 2 and 3 (singles extension) are not supported by this build")
     m = MultTPB(fcidump.norb, adet, bdet, fcidump=fcidump)
     try:
         E, W, _ = m.diag(sbd_data, want_vector=True)
+        Wd = torch.tensor(W, device="cuda")
+        density = m.occupation(Wd)
+        co_a = np.zeros(0, dtype=np.uint64)
+        co_b = np.zeros(0, dtype=np.uint64)
+        if sbd_data.carryover_type == 1:
+            co_a, co_b, _ = m.carryover(Wd, sbd_data.threshold, sbd_data.ratio)
+        del Wd
     finally:
         m.close()
-    return E, W
+    return DiagResult(E, density, co_a, co_b, W)
 
 
 class Server:

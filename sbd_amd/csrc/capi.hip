@@ -74,12 +74,6 @@ void syev_jacobi(int n, double* A, int lda, double* w) {
     for (int i = 0; i < n; i++) A[i + (size_t)j * lda] = V[i + (size_t)idx[j] * n];
 }
 
-}  // namespace sbdx
-
-using namespace sbdx;
-
-namespace {
-
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -88,6 +82,12 @@ int set_err(sbd_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->last_error = msg;
   return code;
 }
+
+}  // namespace sbdx
+
+using namespace sbdx;
+
+namespace {
 
 // SetupIntegrals (makeintegrals.h:124-174): entries in order, later overwrite.
 void setup_integrals(int L, size_t n, const double* val, const int* idx, double& I0,
@@ -652,6 +652,23 @@ int sbd_davidson(sbd_ctx* ctx, const double* hii, double* W, const sbd_options* 
   return rc;
 }
 
+int sbd_lanczos(sbd_ctx* ctx, const double* hii, double* W, const sbd_options* opt,
+                double* energy, double* hist, sbd_report* rep) {
+  if (!ctx || !hii || !W || !opt) return SBD_ERR_ARG;
+  if (!ctx->tables_built) return set_err(ctx, SBD_ERR_STATE, "lanczos before sbd_make_helpers");
+  std::vector<double> h;
+  if (rep) std::memset(rep, 0, sizeof(sbd_report));
+  const double t0 = now_s();
+  int rc = lanczos_core(ctx, hii, W, opt->max_it, opt->max_nb, opt->eps, opt->verbose, energy,
+                        hist ? &h : nullptr, rep, t0);
+  if (hist) {
+    const size_t cap = (size_t)4 * opt->max_it * opt->max_nb;
+    std::memcpy(hist, h.data(), sizeof(double) * std::min(cap, h.size()));
+  }
+  if (rep) rep->t_davidson += now_s() - t0;
+  return rc;
+}
+
 int sbd_diag_footprint(const sbd_ctx* ctx, const sbd_options* opt, uint64_t* bytes) {
   if (!ctx || !opt || !bytes) return SBD_ERR_ARG;
   const uint64_t N = ctx->N;
@@ -684,7 +701,14 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
   if (!ctx || !opt) return SBD_ERR_ARG;
   std::vector<double> hv;
   std::vector<double>* H = hist ? &hv : nullptr;
-  if (opt->method != 0) return set_err(ctx, SBD_ERR_ARG, "only method 0 (Davidson) is supported");
+  // methods 0/1 Davidson, 2/3 Lanczos (sbdiag.h:396-468, 527-600): 1 and 3 are
+  // the reference's stored-Hamiltonian forms of 0 and 2, whose iterates are the
+  // same; here H is never stored
+  if (opt->method < 0 || opt->method > 3)
+    return set_err(ctx, SBD_ERR_ARG, "method must be 0..3 (0/1 Davidson, 2/3 Lanczos)");
+  const bool lanczos = opt->method >= 2;
+  if (lanczos && opt->precision != 0)
+    return set_err(ctx, SBD_ERR_ARG, "precision 1 applies to the Davidson methods (0/1)");
   if (opt->init != 0 && opt->init != 1) return set_err(ctx, SBD_ERR_ARG, "init must be 0 or 1");
   sbd_report local{};
   sbd_report* R = rep ? rep : &local;
@@ -754,8 +778,12 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
     (void)hipFree(W32.p);
     W32.p = nullptr;
   }
-  rc = davidson_core<double>(ctx, hii.p, W.p, opt->max_it, opt->max_nb, opt->eps, opt->max_time,
-                             opt->verbose, &E, H, R, t0);
+  if (lanczos)
+    rc = lanczos_core(ctx, hii.p, W.p, opt->max_it, opt->max_nb, opt->eps, opt->verbose, &E, H,
+                      R, t0);
+  else
+    rc = davidson_core<double>(ctx, hii.p, W.p, opt->max_it, opt->max_nb, opt->eps,
+                               opt->max_time, opt->verbose, &E, H, R, t0);
   if (rc) return rc;
   R->t_davidson = now_s() - t0;
   if (hist && hist_cap > 0) {

@@ -256,6 +256,26 @@ int vec_scale(sbd_ctx* ctx, VT* x, double s, uint64_t n, hipStream_t st);
 template <typename VS, typename VD>
 int vec_convert(sbd_ctx* ctx, const VS* x, VD* y, uint64_t n, hipStream_t st);
 
+// Seconds on a steady clock; records msg as the context's last error and
+// returns code.
+double now_s();
+int set_err(sbd_ctx* ctx, int code, const std::string& msg);
+
+// Restarted Lanczos (chemistry/tpb/lanczos.h:162-327) on device vectors; W is
+// the start vector on entry and sum_ib U[ib, 0] C[ib] of the last pass on exit.
+// hist (optional) receives (E0, beta, seconds since t_start, 64) per step.
+int lanczos_core(sbd_ctx* ctx, const double* hii, double* W, int max_it, int nb, double eps,
+                 int verbose, double* energy, std::vector<double>* hist, sbd_report* rep,
+                 double t_start);
+
+// String weights of a device vector W (na x nb): Da[ia] = sum_ib W^2 and
+// Db[ib] = sum_ia W^2 into host arrays (fixed-order compensated sums).
+int string_weights(sbd_ctx* ctx, const double* W, double* Da_host, double* Db_host,
+                   hipStream_t st);
+// density[2o] / density[2o+1] = alpha / beta occupation of orbital o.
+void occupation_from_weights(const sbd_ctx* ctx, const double* Da, const double* Db,
+                             double* density);
+
 // Scratch of a batching engine (device jobs, partials and results; pinned
 // host results), grown on demand.
 struct BatchScratch {

@@ -147,6 +147,123 @@ def test_oracle_davidson_reaches_dense_ground_state(O, L, na, nb, seed):
     assert abs(abs(np.dot(V[:, 0], W)) - 1.0) < 1e-9
 
 
+@pytest.mark.parametrize("L,na,nb,seed,k", [(6, 3, 3, 19, 6), (6, 2, 3, 5, 8)])
+def test_oracle_lanczos_tridiagonal_is_krylov_projection(O, L, na, nb, seed, k):
+    """One Lanczos pass (lanczos.h:162-327 restated) of k steps from a start
+    vector builds the projection of H on the Krylov space K_k(H, v): the
+    eigenvalues of its tridiagonal (alpha, beta) equal those of Q^T H Q, with Q
+    an orthonormal Krylov basis built here from the dense H by QR (no shared
+    code), and alpha_0 = <v, H v>, beta_0 = |H v - alpha_0 v|."""
+    from oracle import brute
+    from sbd_amd import synth
+    _, h, eri = synth.make_integrals(L, seed, 2 * L)
+    a, b = synth.all_strings(L, na), synth.all_strings(L, nb)
+    S = O.System.from_workload(dict(norb=L, I0=0.5, h=h, eri=eri, a=a, b=b))
+    hii = S.hdiag()
+    v = synth.random_init_vector(S.N, 17)
+    _, hist = S.lanczos(hii, v, 1, k, 1e-300)
+    assert len(hist) == k
+    alpha, beta = hist[:, 1], hist[:-1, 2]
+    Tm = np.diag(alpha) + np.diag(beta, 1) + np.diag(beta, -1)
+    Hb = brute.dense_hamiltonian(L, 0.5, h, synth.unpack_eri(eri, L), a, b)
+    K = np.empty((S.N, k))
+    x = v.copy()
+    for j in range(k):
+        K[:, j] = x
+        x = Hb @ x
+        x /= np.linalg.norm(x)
+    Q, _ = np.linalg.qr(K)
+    Q, _ = np.linalg.qr(Q - Q @ (Q.T @ Q - np.eye(k)) / 2)
+    ev_k = np.linalg.eigvalsh(Q.T @ Hb @ Q)
+    ev_t = np.linalg.eigvalsh(Tm)
+    scale = max(1.0, np.abs(ev_k).max())
+    assert np.abs(ev_t - ev_k).max() < 1e-9 * scale
+    assert abs(alpha[0] - v @ Hb @ v) < 1e-12 * scale
+    assert abs(beta[0] - np.linalg.norm(Hb @ v - alpha[0] * v)) < 1e-12 * scale
+    # the history's E0 is the lowest eigenvalue of the tridiagonal so far
+    assert abs(hist[-1, 0] - ev_t[0]) < 1e-12 * scale
+
+
+@pytest.mark.parametrize("L,na,nb,seed", [(6, 3, 3, 19), (7, 3, 2, 23)])
+def test_oracle_lanczos_reaches_dense_ground_state(O, L, na, nb, seed):
+    """Restarted Lanczos approaches the lowest eigenvalue of the dense H.  Its stop
+    test is the change of E0 between steps (lanczos.h:217-220) and it keeps no
+    orthogonality between passes, so it stalls around 1e-6 Ha here: the bound is
+    1e-4 Ha at eps 1e-6, and the final vector is the ground state to 1e-5."""
+    from oracle import brute
+    from sbd_amd import synth
+    _, h, eri = synth.make_integrals(L, seed, 2 * L)
+    a, b = synth.all_strings(L, na), synth.all_strings(L, nb)
+    S = O.System.from_workload(dict(norb=L, I0=0.5, h=h, eri=eri, a=a, b=b))
+    hii = S.hdiag()
+    W0 = synth.random_init_vector(S.N, 31)
+    W, hist = S.lanczos(hii, W0, 100, 10, 1e-6)
+    assert hist[-1][2] == 0.0  # stopped by the energy test, not by the iteration limit
+    Hb = brute.dense_hamiltonian(L, 0.5, h, synth.unpack_eri(eri, L), a, b)
+    ev, V = np.linalg.eigh(Hb)
+    assert abs(hist[-1][0] - ev[0]) < 1e-4
+    Wn = W / np.linalg.norm(W)
+    assert abs(S.energy(hii, Wn) - ev[0]) < 1e-4
+    assert abs(abs(np.dot(V[:, 0], Wn)) - 1.0) < 1e-5
+
+
+def test_oracle_occupation_and_weights(O):
+    """String weights (rdmat.h:11-58, 136-182) and occupations (occupation.h:
+    10-95) on a random normalised W: the weights sum to |W|^2 = 1, each spin's
+    occupations sum to its electron count (every string holds n electrons), and
+    the occupations equal an independent numpy contraction of W^2 with the
+    strings' bits."""
+    from sbd_amd import synth
+    L, na, nb = 7, 3, 2
+    _, h, eri = synth.make_integrals(L, 5, 2 * L)
+    a, b = synth.all_strings(L, na), synth.all_strings(L, nb)
+    S = O.System.from_workload(dict(norb=L, I0=0.0, h=h, eri=eri, a=a, b=b))
+    W = synth.random_init_vector(S.N, 3)
+    Da, Db = S.weights(W)
+    assert abs(Da.sum() - 1.0) < 1e-13 and abs(Db.sum() - 1.0) < 1e-13
+    W2 = (W * W).reshape(len(a), len(b))
+    assert np.allclose(Da, W2.sum(axis=1), rtol=0, atol=1e-15)
+    assert np.allclose(Db, W2.sum(axis=0), rtol=0, atol=1e-15)
+    occ = S.occupation(W)
+    abits = ((np.asarray(a, dtype=np.uint64)[:, None] >> np.arange(L, dtype=np.uint64)) & 1)
+    bbits = ((np.asarray(b, dtype=np.uint64)[:, None] >> np.arange(L, dtype=np.uint64)) & 1)
+    ref_a = W2.sum(axis=1) @ abits.astype(float)
+    ref_b = W2.sum(axis=0) @ bbits.astype(float)
+    assert np.abs(occ[0::2] - ref_a).max() < 1e-14
+    assert np.abs(occ[1::2] - ref_b).max() < 1e-14
+    assert abs(occ[0::2].sum() - na) < 1e-13 and abs(occ[1::2].sum() - nb) < 1e-13
+
+
+def test_oracle_carryover_rule(O):
+    """CarryOverAdet/Bdet (rdmat.h:60-134) on a W built with known string
+    weights: alpha weights 0.5, 0.3, 0.15, 0.05 (strings 2, 0, 3, 1) and beta
+    0.6, 0.4 (strings 1, 0).  Threshold 0.1 keeps the strings up to and
+    including the first whose cumulative weight exceeds 0.9 (0.5, 0.8, 0.95 ->
+    three); threshold 0.3 (target 0.7) keeps two; ratio 0.5 keeps two alpha
+    (truncated 0.2) and one beta (truncated 0.4)."""
+    from sbd_amd import synth
+    L = 4
+    a = np.array([0b0011, 0b0101, 0b0110, 0b1001], dtype=np.uint64)
+    b = np.array([0b0001, 0b0010], dtype=np.uint64)
+    _, h, eri = synth.make_integrals(L, 1, 2 * L)
+    S = O.System.from_workload(dict(norb=L, I0=0.0, h=h, eri=eri, a=a, b=b))
+    wa = np.array([0.3, 0.05, 0.5, 0.15])
+    wb = np.array([0.4, 0.6])
+    W = np.sqrt(np.outer(wa, wb)).reshape(-1)   # weights factorise: |W|^2 = wa x wb
+    idx, tr = S.carryover(W, 0, 0.1)
+    assert list(idx) == [2, 0, 3] and abs(tr - 0.05) < 1e-15
+    idx, _ = S.carryover(W, 0, 0.3)
+    assert list(idx) == [2, 0]
+    idx, tr = S.carryover(W, 0, 0.0, ratio=0.5)
+    assert list(idx) == [2, 0] and abs(tr - 0.2) < 1e-15
+    idx, tr = S.carryover(W, 1, 0.0, ratio=0.5)
+    assert list(idx) == [1] and abs(tr - 0.4) < 1e-15
+    idx, _ = S.carryover(W, 1, 0.5)            # target 0.5: 0.6 > 0.5 at the first
+    assert list(idx) == [1]
+    idx, _ = S.carryover(W, 1, -1.0)           # target 2: never exceeded -> all strings
+    assert list(idx) == [1, 0]
+
+
 @pytest.mark.parametrize("mol,alpha,E_known", [
     ("h2o", "h2o-1em3-alpha.txt.gz", -76.23594663),
     ("n2", "1em3-alpha.txt.gz", -109.04162110),
