@@ -31,6 +31,10 @@
  *                                                        sbd_occupation
  *   sbd::CarryOverAdet / CarryOverBdet chemistry/tpb/rdmat.h:60-134 (+ beta)
  *                                                        sbd_carryover
+ *   sbd::SaveWavefunction / LoadWavefunction chemistry/tpb/restart.h:20-65,
+ *                        414-657                         sbd_save_wavefunction,
+ *                                                        sbd_load_wavefunction
+ *   diag with loadname   chemistry/tpb/sbdiag.h:339-347  sbd_diag_from
  *
  * Conventions
  *   - Strings are packed one uint64 per alpha/beta string (orbital p -> bit p);
@@ -192,6 +196,23 @@ int sbd_lanczos(sbd_ctx* ctx, const double* hii, double* W, const sbd_options* o
  * (E0, |R|, seconds since the solver started, 32 or 64 = vector precision). */
 int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out, double* hist,
              int hist_cap, sbd_report* rep);
+
+/* sbd_diag from a given start vector W_init (host, N doubles; opt->init is
+ * ignored): the warm start of sbd::tpb::diag with a loadname
+ * (tpb/sbdiag.h:339-347), e.g. from sbd_load_wavefunction. */
+int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, double* energy,
+                  double* W_out, double* hist, int hist_cap, sbd_report* rep);
+
+/* Wavefunction files (tpb/restart.h:20-65, 414-657) at one rank: the file is
+ * prefix + "000000" and holds size_t na, nb, words per string, the alpha then
+ * beta strings as det_vector words of bit_length bits, then W (host, N
+ * doubles, W[ia * nb + ib]).  Loading places each saved coefficient whose
+ * alpha and beta strings are in this context's basis at their position here
+ * (others are dropped) and normalises W; *placed (optional) = coefficients
+ * placed (0: W is all zeros). */
+int sbd_save_wavefunction(sbd_ctx* ctx, const char* prefix, size_t bit_length, const double* W);
+int sbd_load_wavefunction(sbd_ctx* ctx, const char* prefix, size_t bit_length, double* W,
+                          uint64_t* placed);
 
 /* The nroots lowest states by block Davidson (Davidson-Liu, restarted GD+k
  * style from the current and previous Ritz vectors): sbd::tpb::diag

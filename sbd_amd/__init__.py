@@ -32,7 +32,8 @@ EXPORTS = [
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
     "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
-    "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_carryover",
+    "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_carryover", "sbd_diag_from",
+    "sbd_save_wavefunction", "sbd_load_wavefunction",
     "sbd_diag", "sbd_diag_roots", "sbd_diag_footprint", "sbd_server_create", "sbd_server_submit",
     "sbd_server_wait", "sbd_server_poll", "sbd_server_now", "sbd_server_stats",
     "sbd_server_destroy",
@@ -112,6 +113,10 @@ def lib():
     L.sbd_davidson.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_lanczos.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_string_weights.argtypes = [vp, vp, dp, dp, vp]
+    L.sbd_diag_from.argtypes = [vp, C.POINTER(Options), dp, dp, dp, dp, C.c_int,
+                                C.POINTER(Report)]
+    L.sbd_save_wavefunction.argtypes = [vp, C.c_char_p, C.c_size_t, dp]
+    L.sbd_load_wavefunction.argtypes = [vp, C.c_char_p, C.c_size_t, dp, up]
     L.sbd_occupation.argtypes = [vp, vp, dp, vp]
     L.sbd_carryover.argtypes = [vp, vp, C.c_double, C.c_double, up, szp, up, szp, dp, vp]
     L.sbd_diag.argtypes = [vp, C.POINTER(Options), dp, dp, dp, C.c_int, C.POINTER(Report)]
@@ -361,7 +366,22 @@ class MultTPB:
             out = out + (hist[:4 * rep.steps].reshape(-1, 4),)
         return out
 
-    def diag(self, sbd: SBD, want_vector=False, history=False):
+    def save_wavefunction(self, prefix, W, bit_length=20):
+        """SaveWavefunction (restart.h:20-65): writes prefix + '000000'."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        _check(lib().sbd_save_wavefunction(self._ctx, str(prefix).encode(), int(bit_length),
+                                           _ptr(W, C.c_double)), self._ctx)
+
+    def load_wavefunction(self, prefix, bit_length=20):
+        """LoadWavefunction (restart.h:414-657): the saved coefficients mapped
+        onto this basis and normalised; returns (W, number placed)."""
+        W = np.zeros(self.N)
+        placed = C.c_uint64(0)
+        _check(lib().sbd_load_wavefunction(self._ctx, str(prefix).encode(), int(bit_length),
+                                           _ptr(W, C.c_double), C.byref(placed)), self._ctx)
+        return W, placed.value
+
+    def diag(self, sbd: SBD, want_vector=False, history=False, W_init=None):
         """sbd::tpb::diag: method 0/1 Davidson, 2/3 Lanczos.  Returns (energy,
         W or None, report[, history]); Davidson history rows are (E0, |R|,
         seconds since solver start, vector precision 32/64), Lanczos rows
@@ -372,9 +392,17 @@ class MultTPB:
         W = np.zeros(self.N) if want_vector else None
         cap = sbd.max_it * sbd.max_nb * (2 if sbd.precision == 1 else 1)
         hist = np.zeros(4 * cap)
-        _check(lib().sbd_diag(self._ctx, C.byref(o), C.byref(E),
-                              _ptr(W, C.c_double) if want_vector else None,
-                              _ptr(hist, C.c_double), cap, C.byref(rep)), self._ctx)
+        if W_init is not None:
+            W0 = np.ascontiguousarray(W_init, dtype=np.float64)
+            if W0.size != self.N:
+                raise SbdError("W_init must have N elements")
+            _check(lib().sbd_diag_from(self._ctx, C.byref(o), _ptr(W0, C.c_double), C.byref(E),
+                                       _ptr(W, C.c_double) if want_vector else None,
+                                       _ptr(hist, C.c_double), cap, C.byref(rep)), self._ctx)
+        else:
+            _check(lib().sbd_diag(self._ctx, C.byref(o), C.byref(E),
+                                  _ptr(W, C.c_double) if want_vector else None,
+                                  _ptr(hist, C.c_double), cap, C.byref(rep)), self._ctx)
         out = (E.value, W, rep.as_dict())
         if history:
             out = out + (hist[:4 * min(rep.steps, cap)].reshape(-1, 4),)
@@ -452,8 +480,6 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
     the occupation density and, with carryover_type 1, the carry-over strings
     (sbdiag.h:666-777).  Returns a DiagResult."""
     import torch
-    if loadname or savename:
-        raise SbdError("wavefunction load/save is not supported by this build")
     if sbd_data.do_rdm != 0:
         raise SbdError("do_rdm != 0 (full one- and two-particle RDMs) is not supported by this "
                        "build; do_rdm = 0 gives the occupation density")
@@ -461,7 +487,15 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
         raise SbdError("carryover_type 2 and 3 (singles extension) are not supported by this build")
     m = MultTPB(fcidump.norb, adet, bdet, fcidump=fcidump)
     try:
-        E, W, _ = m.diag(sbd_data, want_vector=True)
+        W0 = None
+        if loadname:
+            # initial vector from a saved wavefunction (sbdiag.h:339-347)
+            W0, placed = m.load_wavefunction(loadname, sbd_data.bit_length)
+            if placed == 0:
+                raise SbdError(f"{loadname}: no saved coefficient lies in this basis")
+        E, W, _ = m.diag(sbd_data, want_vector=True, W_init=W0)
+        if savename:
+            m.save_wavefunction(savename, W, sbd_data.bit_length)
         Wd = torch.tensor(W, device="cuda")
         density = m.occupation(Wd)
         co_a = np.zeros(0, dtype=np.uint64)

@@ -698,6 +698,11 @@ int sbd_diag_footprint(const sbd_ctx* ctx, const sbd_options* opt, uint64_t* byt
 
 int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out, double* hist,
              int hist_cap, sbd_report* rep) {
+  return sbd_diag_from(ctx, opt, nullptr, energy, W_out, hist, hist_cap, rep);
+}
+
+int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, double* energy,
+                  double* W_out, double* hist, int hist_cap, sbd_report* rep) {
   if (!ctx || !opt) return SBD_ERR_ARG;
   std::vector<double> hv;
   std::vector<double>* H = hist ? &hv : nullptr;
@@ -719,9 +724,12 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
   if (rc) return rc;
   SBD_HIP_TRY(ctx, hipDeviceSynchronize());
   R->t_tables = now_s() - t0;
-  // initial vector (BasisInitVector, davidson.h:144-184)
+  // initial vector (BasisInitVector, davidson.h:144-184), or the caller's
+  // (a loaded wavefunction, sbdiag.h:339-347)
   std::vector<double> W0(N, 0.0);
-  if (opt->init == 0) {
+  if (W_init) {
+    std::memcpy(W0.data(), W_init, 8 * N);
+  } else if (opt->init == 0) {
     W0[0] = 1.0;
   } else {
     // Randomize (framework/dm_vector.h:101-191 semantics: uniform(-1,1), then
