@@ -35,6 +35,15 @@
  *                        414-657                         sbd_save_wavefunction,
  *                                                        sbd_load_wavefunction
  *   diag with loadname   chemistry/tpb/sbdiag.h:339-347  sbd_diag_from
+ *   sbd::SinglesExtendHalfdets chemistry/tpb/extend.h:14-185 (amplitude form)
+ *                                                        sbd_singles_extend
+ *                        chemistry/tpb/extend.h:186-324 (string form)
+ *                                                        sbd_singles_extend_strings
+ *   sbd::sort_bitarray   framework/bit_manipulation.h:421-449 sbd_sort_strings
+ *   sbd::DecodeAlphaDets chemistry/basic/makedeterminants.h:61-114
+ *                                                        sbd_decode_strings_bin
+ *   carryover.bin writer apps/chemistry_tpb_selected_basis_diagonalization/
+ *                        main.cc:245-260                 sbd_encode_strings_bin
  *
  * Conventions
  *   - Strings are packed one uint64 per alpha/beta string (orbital p -> bit p);
@@ -251,6 +260,33 @@ int sbd_occupation(sbd_ctx* ctx, const double* W, double* density, void* stream)
 int sbd_carryover(sbd_ctx* ctx, const double* W, double threshold, double ratio,
                   uint64_t* a_out, size_t* na_out, uint64_t* b_out, size_t* nb_out,
                   double* truncated, void* stream);
+
+/* Singles extension (carry-over types 2 and 3 of tpb/sbdiag.h:773-791).
+ * Amplitude form (SinglesExtendHalfdets, tpb/extend.h:14-185): the strings
+ * of the determinants with W^2 > cutoff (device W), each with all of its
+ * single excitations, sorted ascending without duplicates; *total_weight =
+ * the weight of those determinants.  String form (tpb/extend.h:186-324): the
+ * same extension of given strings.  Outputs are host arrays of capacity cap_*;
+ * a too small capacity is SBD_ERR_ARG with the needed size in *n*_out. */
+int sbd_singles_extend(sbd_ctx* ctx, const double* W, double cutoff, uint64_t* a_out,
+                       size_t cap_a, size_t* na_out, uint64_t* b_out, size_t cap_b,
+                       size_t* nb_out, double* total_weight, void* stream);
+int sbd_singles_extend_strings(int norb, const uint64_t* in, size_t n, uint64_t* out, size_t cap,
+                               size_t* n_out);
+/* sort_bitarray (framework/bit_manipulation.h:421-449): ascending, duplicates
+ * removed, in place; *n_out = strings kept. */
+int sbd_sort_strings(uint64_t* s, size_t n, size_t* n_out);
+/* Binary string files of ceil(norb/8) bytes per string.  Decoding follows
+ * DecodeAlphaDets (chemistry/basic/makedeterminants.h:61-114): the record is a
+ * big-endian number whose low norb bits are the string (orbital p = bit p);
+ * strings come out in file order, unsorted.  Encoding follows the
+ * carryover.bin writer (main.cc:245-260): orbital norb-1 is the first bit of
+ * the record and the padding bits are the last, so for norb % 8 != 0 the
+ * reference's writer and reader disagree; both are kept as the reference has
+ * them.  Decoding into too small a capacity is SBD_ERR_ARG with *n_out = the
+ * number of strings in the file. */
+int sbd_decode_strings_bin(const char* path, int norb, uint64_t* out, size_t cap, size_t* n_out);
+int sbd_encode_strings_bin(const char* path, int norb, const uint64_t* s, size_t n);
 
 /* Device bytes sbd_diag would allocate for these options (vectors + tables +
  * integrals), for admission control. */

@@ -33,7 +33,9 @@ EXPORTS = [
     "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
     "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_carryover", "sbd_diag_from",
-    "sbd_save_wavefunction", "sbd_load_wavefunction",
+    "sbd_save_wavefunction", "sbd_load_wavefunction", "sbd_singles_extend",
+    "sbd_singles_extend_strings", "sbd_sort_strings", "sbd_decode_strings_bin",
+    "sbd_encode_strings_bin",
     "sbd_diag", "sbd_diag_roots", "sbd_diag_footprint", "sbd_server_create", "sbd_server_submit",
     "sbd_server_wait", "sbd_server_poll", "sbd_server_now", "sbd_server_stats",
     "sbd_server_destroy",
@@ -113,6 +115,12 @@ def lib():
     L.sbd_davidson.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_lanczos.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_string_weights.argtypes = [vp, vp, dp, dp, vp]
+    L.sbd_singles_extend.argtypes = [vp, vp, C.c_double, up, C.c_size_t, szp, up, C.c_size_t,
+                                     szp, dp, vp]
+    L.sbd_singles_extend_strings.argtypes = [C.c_int, up, C.c_size_t, up, C.c_size_t, szp]
+    L.sbd_sort_strings.argtypes = [up, C.c_size_t, szp]
+    L.sbd_decode_strings_bin.argtypes = [C.c_char_p, C.c_int, up, C.c_size_t, szp]
+    L.sbd_encode_strings_bin.argtypes = [C.c_char_p, C.c_int, up, C.c_size_t]
     L.sbd_diag_from.argtypes = [vp, C.POINTER(Options), dp, dp, dp, dp, C.c_int,
                                 C.POINTER(Report)]
     L.sbd_save_wavefunction.argtypes = [vp, C.c_char_p, C.c_size_t, dp]
@@ -164,7 +172,7 @@ class SBD:
     seed: int = 1729
     bit_length: int = 20
     do_rdm: int = 0              # 0: occupation density only (full RDMs not built)
-    carryover_type: int = 0      # 0 none, 1 dominant strings
+    carryover_type: int = 0      # 0 none, 1 dominant strings, 2 + singles, 3 singles of W^2 > threshold
     ratio: float = 0.0
     threshold: float = 0.01
     precision: int = 0           # added: 1 = FP32 vectors + FP64 correction
@@ -240,6 +248,50 @@ def pack_strings(words: np.ndarray, bit_length: int, norb: int) -> np.ndarray:
     _check(lib().sbd_pack_strings(words.ctypes.data_as(C.POINTER(C.c_size_t)), n, bit_length, norb,
                                   _ptr(out, C.c_uint64)))
     return out
+
+
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def sort_bitarray(strings):
+    """sort_bitarray (bit_manipulation.h:421-449): ascending, unique."""
+    s = _u64(strings).copy()
+    n = C.c_size_t(0)
+    _check(lib().sbd_sort_strings(_ptr(s, C.c_uint64), len(s), C.byref(n)))
+    return s[:n.value]
+
+
+def singles_extend_strings(strings, norb):
+    """SinglesExtendHalfdets, string form (extend.h:186-324): the strings and
+    all their single excitations, sorted and unique."""
+    s = _u64(strings)
+    cap = max(1, len(s) * (1 + norb * norb // 4))
+    out = np.zeros(cap, dtype=np.uint64)
+    n = C.c_size_t(0)
+    _check(lib().sbd_singles_extend_strings(int(norb), _ptr(s, C.c_uint64), len(s),
+                                            _ptr(out, C.c_uint64), cap, C.byref(n)))
+    return out[:n.value].copy()
+
+
+def DecodeAlphaDets(path, norb):
+    """DecodeAlphaDets (makedeterminants.h:61-114): strings of a binary file in
+    file order (unsorted; the CLI sorts them with sort_bitarray)."""
+    n = C.c_size_t(0)
+    rc = lib().sbd_decode_strings_bin(str(path).encode(), int(norb), None, 0, C.byref(n))
+    if rc not in (0, 1):
+        _check(rc)
+    out = np.zeros(max(1, n.value), dtype=np.uint64)
+    _check(lib().sbd_decode_strings_bin(str(path).encode(), int(norb), _ptr(out, C.c_uint64),
+                                        len(out), C.byref(n)))
+    return out[:n.value].copy()
+
+
+def encode_strings_bin(path, strings, norb):
+    """The carryover.bin writer (main.cc:245-260)."""
+    s = _u64(strings)
+    _check(lib().sbd_encode_strings_bin(str(path).encode(), int(norb), _ptr(s, C.c_uint64),
+                                        len(s)))
 
 
 class MultTPB:
@@ -456,6 +508,20 @@ class MultTPB:
                                    C.byref(nb), _ptr(tr, C.c_double), _stream()), self._ctx)
         return ao[:na.value].copy(), bo[:nb.value].copy(), tr
 
+    def singles_extend(self, W, cutoff):
+        """SinglesExtendHalfdets, amplitude form (extend.h:14-185): (alpha
+        strings, beta strings, weight of the determinants above the cutoff)."""
+        L = self.norb
+        ca = max(1, self.na * (1 + L * L // 4))
+        cb = max(1, self.nb * (1 + L * L // 4))
+        ao, bo = np.zeros(ca, dtype=np.uint64), np.zeros(cb, dtype=np.uint64)
+        na, nb = C.c_size_t(0), C.c_size_t(0)
+        tw = C.c_double(0.0)
+        _check(lib().sbd_singles_extend(self._ctx, _dptr(W), float(cutoff), _ptr(ao, C.c_uint64),
+                                        ca, C.byref(na), _ptr(bo, C.c_uint64), cb, C.byref(nb),
+                                        C.byref(tw), _stream()), self._ctx)
+        return ao[:na.value].copy(), bo[:nb.value].copy(), tw.value
+
     def footprint(self, sbd: SBD):
         o = sbd.options()
         out = C.c_uint64(0)
@@ -483,8 +549,8 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
     if sbd_data.do_rdm != 0:
         raise SbdError("do_rdm != 0 (full one- and two-particle RDMs) is not supported by this "
                        "build; do_rdm = 0 gives the occupation density")
-    if sbd_data.carryover_type not in (0, 1):
-        raise SbdError("carryover_type 2 and 3 (singles extension) are not supported by this build")
+    if sbd_data.carryover_type not in (0, 1, 2, 3):
+        raise SbdError("carryover_type must be 0..3")
     m = MultTPB(fcidump.norb, adet, bdet, fcidump=fcidump)
     try:
         W0 = None
@@ -500,8 +566,16 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
         density = m.occupation(Wd)
         co_a = np.zeros(0, dtype=np.uint64)
         co_b = np.zeros(0, dtype=np.uint64)
-        if sbd_data.carryover_type == 1:
+        # carry-over strings (sbdiag.h:745-791): 1 dominant strings, 2 those
+        # extended by singles, 3 singles extension of the strings of the
+        # determinants above the threshold
+        if sbd_data.carryover_type in (1, 2):
             co_a, co_b, _ = m.carryover(Wd, sbd_data.threshold, sbd_data.ratio)
+            if sbd_data.carryover_type == 2:
+                co_a = singles_extend_strings(co_a, m.norb)
+                co_b = singles_extend_strings(co_b, m.norb)
+        elif sbd_data.carryover_type == 3:
+            co_a, co_b, _ = m.singles_extend(Wd, sbd_data.threshold)
         del Wd
     finally:
         m.close()

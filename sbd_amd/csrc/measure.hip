@@ -16,6 +16,7 @@
 // determinant).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <numeric>
 #include <vector>
 
@@ -182,11 +183,193 @@ void occupation_from_weights(const sbd_ctx* ctx, const double* Da, const double*
   for (int o = 0; o < 2 * L; o++) density[o] = acc[o].s;
 }
 
+namespace {
+
+// Strings of the determinants whose weight exceeds a cutoff
+// (SinglesExtendHalfdets, chemistry/tpb/extend.h:14-58): rowflag[ia] /
+// colflag[ib] set when any determinant of that string has W^2 > cutoff; the
+// weight above the cutoff per row in rowsum (summed on the host in row order).
+__global__ __launch_bounds__(kWBlock) void k_cut_flags(const double* __restrict__ W, uint64_t na,
+                                                       uint64_t nb, double cutoff,
+                                                       uint32_t* __restrict__ rowflag,
+                                                       uint32_t* __restrict__ colflag,
+                                                       double* __restrict__ rowsum) {
+  __shared__ double red[kWBlock];
+  __shared__ uint32_t any;
+  for (uint64_t ia = blockIdx.x; ia < na; ia += gridDim.x) {
+    __syncthreads();
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    const double* __restrict__ row = W + ia * nb;
+    Kahan2 k;
+    uint32_t f = 0;
+    for (uint64_t ib = threadIdx.x; ib < nb; ib += kWBlock) {
+      const double w2 = row[ib] * row[ib];
+      if (w2 > cutoff) {
+        k.add(w2);
+        f = 1;
+        colflag[ib] = 1;  // only ever set to 1: the result does not depend on order
+      }
+    }
+    if (f) any = 1;
+    red[threadIdx.x] = k.s;
+    __syncthreads();
+    for (int w = kWBlock / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      rowflag[ia] = any;
+      rowsum[ia] = red[0];
+    }
+  }
+}
+
+// Each string and all of its single excitations (single_from_hdet over the
+// occupied x unoccupied orbitals), then sorted ascending without duplicates
+// (sort_bitarray, framework/bit_manipulation.h:421-449).
+std::vector<uint64_t> extend_singles(const std::vector<uint64_t>& in, int norb) {
+  std::vector<uint64_t> out;
+  out.reserve(in.size() * (1 + (size_t)norb * norb / 4));
+  const uint64_t full = norb >= 64 ? ~0ULL : ((1ULL << norb) - 1ULL);
+  for (uint64_t s : in) {
+    out.push_back(s);
+    for (uint64_t occ = s; occ; occ &= occ - 1) {
+      const uint64_t i = occ & (~occ + 1);
+      for (uint64_t vir = ~s & full; vir; vir &= vir - 1) {
+        const uint64_t a = vir & (~vir + 1);
+        out.push_back((s ^ i) | a);
+      }
+    }
+  }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
+int copy_out(const std::vector<uint64_t>& v, uint64_t* out, size_t cap, size_t* n_out) {
+  *n_out = v.size();
+  if (v.size() > cap) return SBD_ERR_ARG;
+  std::copy(v.begin(), v.end(), out);
+  return SBD_OK;
+}
+
+}  // namespace
+
 }  // namespace sbdx
 
 using namespace sbdx;
 
 extern "C" {
+
+int sbd_singles_extend_strings(int norb, const uint64_t* in, size_t n, uint64_t* out, size_t cap,
+                               size_t* n_out) {
+  if (norb < 1 || norb > 64 || (n && !in) || !out || !n_out) return SBD_ERR_ARG;
+  return copy_out(extend_singles(std::vector<uint64_t>(in, in + n), norb), out, cap, n_out);
+}
+
+int sbd_singles_extend(sbd_ctx* ctx, const double* W, double cutoff, uint64_t* a_out,
+                       size_t cap_a, size_t* na_out, uint64_t* b_out, size_t cap_b,
+                       size_t* nb_out, double* total_weight, void* stream) {
+  if (!ctx || !W || !a_out || !na_out || !b_out || !nb_out) return SBD_ERR_ARG;
+  const uint64_t na = ctx->na, nb = ctx->nb;
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t *rf = nullptr, *cf = nullptr;
+  double* rs = nullptr;
+  auto cleanup = [&]() {
+    if (rf) (void)hipFree(rf);
+    if (cf) (void)hipFree(cf);
+    if (rs) (void)hipFree(rs);
+  };
+  if (hipMalloc(&rf, 4 * na) != hipSuccess || hipMalloc(&cf, 4 * nb) != hipSuccess ||
+      hipMalloc(&rs, 8 * na) != hipSuccess) {
+    cleanup();
+    return set_err(ctx, SBD_ERR_MEMORY, "device allocation for singles extension failed");
+  }
+  std::vector<uint32_t> hrf(na), hcf(nb);
+  std::vector<double> hrs(na);
+  int rc = SBD_OK;
+  if (hipMemsetAsync(cf, 0, 4 * nb, st) != hipSuccess) rc = SBD_ERR_DEVICE;
+  if (!rc) {
+    hipLaunchKernelGGL(k_cut_flags, dim3((unsigned)std::min<uint64_t>(na, 65535)), dim3(kWBlock),
+                       0, st, W, na, nb, cutoff, rf, cf, rs);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(hrf.data(), rf, 4 * na, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(hcf.data(), cf, 4 * nb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(hrs.data(), rs, 8 * na, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = SBD_ERR_DEVICE;
+  }
+  cleanup();
+  if (rc) return set_err(ctx, rc, "singles extension: device error");
+  std::vector<uint64_t> ka, kb;
+  double tw = 0.0;
+  for (uint64_t i = 0; i < na; i++) {
+    tw += hrs[i];
+    if (hrf[i]) ka.push_back(ctx->a_host[i]);
+  }
+  for (uint64_t i = 0; i < nb; i++)
+    if (hcf[i]) kb.push_back(ctx->b_host[i]);
+  if (total_weight) *total_weight = tw;
+  rc = copy_out(extend_singles(ka, ctx->norb), a_out, cap_a, na_out);
+  const int rc2 = copy_out(extend_singles(kb, ctx->norb), b_out, cap_b, nb_out);
+  if (rc || rc2) return set_err(ctx, SBD_ERR_ARG, "singles extension: output capacity too small");
+  return SBD_OK;
+}
+
+int sbd_sort_strings(uint64_t* s, size_t n, size_t* n_out) {
+  if ((n && !s) || !n_out) return SBD_ERR_ARG;
+  std::sort(s, s + n);
+  *n_out = (size_t)(std::unique(s, s + n) - s);
+  return SBD_OK;
+}
+
+int sbd_decode_strings_bin(const char* path, int norb, uint64_t* out, size_t cap,
+                           size_t* n_out) {
+  if (!path || norb < 1 || norb > 64 || !n_out) return SBD_ERR_ARG;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return SBD_ERR_IO;
+  const size_t nbytes = (norb + 7) / 8, pad = 8 * nbytes - norb;
+  std::vector<uint8_t> rec(nbytes);
+  size_t n = 0;
+  int rc = SBD_OK;
+  while (std::fread(rec.data(), 1, nbytes, f) == nbytes) {
+    // bits most significant first; the first `pad` bits are skipped; the last
+    // bit read is orbital 0 (DecodeAlphaDetsBits, makedeterminants.h:61-103)
+    uint64_t v = 0;
+    size_t bit = 0;
+    for (size_t k = 0; k < nbytes; k++)
+      for (int i = 7; i >= 0; i--, bit++)
+        if (bit >= pad) v = (v << 1) | ((rec[k] >> i) & 1u);
+    if (n < cap && out) out[n] = v;
+    else rc = SBD_ERR_ARG;
+    n++;
+  }
+  std::fclose(f);
+  *n_out = n;
+  return rc;
+}
+
+int sbd_encode_strings_bin(const char* path, int norb, const uint64_t* s, size_t n) {
+  if (!path || norb < 1 || norb > 64 || (n && !s)) return SBD_ERR_ARG;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return SBD_ERR_IO;
+  const size_t nbytes = (norb + 7) / 8;
+  std::vector<uint8_t> rec(nbytes);
+  bool ok = true;
+  for (size_t q = 0; q < n; q++) {
+    // orbital norb - 1 - j at bit j of the record, most significant bit of
+    // byte 0 first, padding at the end (the carryover.bin writer of
+    // apps/chemistry_tpb_selected_basis_diagonalization/main.cc:245-260)
+    std::fill(rec.begin(), rec.end(), 0);
+    for (int j = 0; j < norb; j++)
+      if ((s[q] >> (norb - 1 - j)) & 1ULL) rec[j / 8] |= (uint8_t)(1u << (7 - j % 8));
+    ok = ok && std::fwrite(rec.data(), 1, nbytes, f) == nbytes;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? SBD_OK : SBD_ERR_IO;
+}
+
 
 int sbd_string_weights(sbd_ctx* ctx, const double* W, double* wa, double* wb, void* stream) {
   if (!ctx || !W || !wa || !wb) return SBD_ERR_ARG;

@@ -92,3 +92,27 @@ def test_weights_full_size(torch):
     occ = m.occupation(x)
     assert abs(occ[0::2].sum() - 8 * n2) < 1e-11 * n2
     m.close()
+
+
+def test_singles_extension_amplitude_form(torch):
+    """SinglesExtendHalfdets (extend.h:14-185) on a converged sci16 vector:
+    the strings of the determinants with W^2 > cutoff, extended by singles,
+    against a numpy selection and the string form; the weight above the
+    cutoff within 1e-13."""
+    import sbd_amd
+    from sbd_amd import synth
+    say("singles extension sci16")
+    w = synth.make_workload("sci16", 1.0)
+    m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
+    _, W, _ = m.diag(sbd_amd.SBD(max_it=50, max_nb=10, eps=1e-6), want_vector=True)
+    Wd = torch.tensor(W, device="cuda")
+    for cut in (1e-3, 1e-5):
+        ea, eb, tw = m.singles_extend(Wd, cut)
+        W2 = (W * W).reshape(m.na, m.nb)
+        sel = W2 > cut
+        ka = np.asarray(w["a"])[sel.any(axis=1)]
+        kb = np.asarray(w["b"])[sel.any(axis=0)]
+        assert np.array_equal(ea, sbd_amd.singles_extend_strings(ka, m.norb))
+        assert np.array_equal(eb, sbd_amd.singles_extend_strings(kb, m.norb))
+        assert abs(tw - W2[sel].sum()) < 1e-13
+    m.close()

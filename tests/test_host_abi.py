@@ -123,3 +123,54 @@ def test_no_cpu_fallback_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(sbd_amd, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(sbd_amd.SbdError, match="not built"):
         sbd_amd.lib()
+
+
+def _all_strings(L, n):
+    from itertools import combinations
+    return np.array(sorted(sum(1 << p for p in c) for c in combinations(range(L), n)),
+                    dtype=np.uint64)
+
+
+def test_singles_extension_closed_forms(L):
+    """SinglesExtendHalfdets, string form (extend.h:186-324): one string of n
+    electrons in L orbitals extends to 1 + n(L-n) strings, exactly those of
+    Hamming distance 0 or 2 with the same electron count (brute force); the
+    full set of weight-n strings is closed under singles; the output is
+    sorted and unique (sort_bitarray)."""
+    import sbd_amd
+    for Lo, n in ((8, 3), (12, 4), (40, 2)):
+        hf = (1 << n) - 1
+        out = sbd_amd.singles_extend_strings([hf], Lo)
+        assert len(out) == 1 + n * (Lo - n)
+        assert np.all(np.diff(out.astype(np.float64)) > 0)
+        if Lo <= 12:
+            full = _all_strings(Lo, n)
+            dist = np.array([bin(int(s) ^ hf).count("1") for s in full])
+            assert np.array_equal(out, full[dist <= 2])
+            assert np.array_equal(sbd_amd.singles_extend_strings(full, Lo), full)
+    assert np.array_equal(sbd_amd.sort_bitarray([5, 3, 5, 9, 3]), np.array([3, 5, 9],
+                                                                           dtype=np.uint64))
+
+
+def test_binary_strings_kats(L, tmp_path):
+    """DecodeAlphaDets (makedeterminants.h:61-114): a record of ceil(L/8) bytes
+    is a big-endian number whose low L bits are the string (leading padding
+    skipped); the carryover.bin writer (main.cc:245-260) shifts the string to
+    the top of the record (padding last).  For L = 12, string {0, 5, 11} =
+    0x821: decoded from bytes 08 21; written as bytes 82 10."""
+    import sbd_amd
+    p = tmp_path / "s.bin"
+    p.write_bytes(bytes([0x08, 0x21, 0xF0, 0x01]))    # second record: padding bits set
+    got = sbd_amd.DecodeAlphaDets(p, 12)
+    assert list(got) == [0x821, 0x001]
+    sbd_amd.encode_strings_bin(p, [0x821], 12)
+    assert p.read_bytes() == bytes([0x82, 0x10])
+    rng = np.random.default_rng(4)
+    for Lo in (8, 16, 36, 64):
+        s = rng.integers(0, 2 ** min(Lo, 63), size=20, dtype=np.uint64)
+        nb = (Lo + 7) // 8
+        p.write_bytes(b"".join(int(v).to_bytes(nb, "big") for v in s))
+        assert np.array_equal(sbd_amd.DecodeAlphaDets(p, Lo), s)
+        if Lo % 8 == 0:   # writer and reader agree when there is no padding
+            sbd_amd.encode_strings_bin(p, s, Lo)
+            assert np.array_equal(sbd_amd.DecodeAlphaDets(p, Lo), s)
