@@ -15,10 +15,13 @@ Printed JSON line:
                FP64 correction, same workload
   cpu_baseline the CPU oracle (restatement of the reference's OpenMP sigma) on
                a bounded sample of bra rows of the same workload
-  serving      a Poisson stream of independent solves (norb 16-40, n 4-10,
+  rdm          wall time of the full one- and two-particle RDMs (do_rdm != 0)
+               of a normalised vector on the same workload
+  serving      Poisson streams of independent solves (norb 16-40, n 4-10,
                200-8000 strings per spin) through the continuous-batching
                server vs the same stream solved one at a time: p50/p99
-               latency and solves/s
+               latency and solves/s, at overload (64 requests at 40/s) and at
+               a steady sub-saturation load (32 requests at 0.5/s)
 """
 import argparse
 import json
@@ -90,19 +93,11 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
 
 
-def serving(args):
-    """Poisson stream of independent solves (norb 16-40, n 4-10, 200-8000 strings
-    per spin, per-request integrals), served with continuous batching, against the
-    same stream solved one request at a time with the plain solver."""
+def serve_stream(reqs, opt, args, label):
+    """Serve one request stream with continuous batching, then the same stream
+    one request at a time with the plain solver (queueing included)."""
     import threading
     import sbd_amd
-    from sbd_amd import synth
-    t0 = time.time()
-    reqs = synth.serving_requests(args.serve_count, 4004, rate=args.serve_rate)
-    log(f"serving: {len(reqs)} requests generated in {time.time() - t0:.1f}s, "
-        f"N total {sum(len(r['a']) * len(r['b']) for r in reqs):.3e}")
-    opt = sbd_amd.SBD(max_it=args.serve_max_it, max_nb=10, eps=args.serve_eps, init=0)
-    # continuous batching
     sv = sbd_amd.Server(args.serve_mem_gb << 30, max_active=args.serve_slots)
     ids = [None] * len(reqs)
     base = sv.now()
@@ -121,15 +116,14 @@ def serving(args):
     res = []
     for q, i in enumerate(ids):
         res.append(sv.wait(i))
-        if (q + 1) % 64 == 0:
-            log(f"serving: batched, {q + 1} solves done")
+        if (q + 1) % 16 == 0:
+            log(f"serving {label}: batched, {q + 1} solves done")
     st = sv.stats()
     sv.close()
     lat = [r["t_done"] - r["t_submit"] for r in res]
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
-    log(f"serving: batched p50 {pct(lat, 50):.3f}s p99 {pct(lat, 99):.3f}s "
+    log(f"serving {label}: batched p50 {pct(lat, 50):.3f}s p99 {pct(lat, 99):.3f}s "
         f"{len(res) / span:.2f} solves/s peak active {st['peak_active']}")
-    # sequential plain solves of the same stream (queueing included)
     seq_lat, seq_E = [], []
     t_start = time.time()
     for r in reqs:
@@ -141,15 +135,14 @@ def serving(args):
         m.close()
         seq_E.append(E)
         seq_lat.append(time.time() - t_start - r["t"])
-        if len(seq_E) % 32 == 0:
-            log(f"serving: sequential, {len(seq_E)} solves done")
+        if len(seq_E) % 16 == 0:
+            log(f"serving {label}: sequential, {len(seq_E)} solves done")
     seq_span = time.time() - t_start - reqs[0]["t"]
-    log(f"serving: sequential p50 {pct(seq_lat, 50):.3f}s p99 {pct(seq_lat, 99):.3f}s "
+    log(f"serving {label}: sequential p50 {pct(seq_lat, 50):.3f}s p99 {pct(seq_lat, 99):.3f}s "
         f"{len(reqs) / seq_span:.2f} solves/s")
     same = sum(1 for a, b in zip(res, seq_E) if a["energy"] == b)
     return {
-        "requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
-        "options": {"max_it": args.serve_max_it, "max_nb": 10, "eps": args.serve_eps, "init": 0},
+        "requests": len(reqs),
         "batched": {"solves_per_s": round(len(res) / span, 3), "p50_s": round(pct(lat, 50), 4),
                     "p99_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3),
                     "peak_active": st["peak_active"], "rounds": st["rounds"],
@@ -159,8 +152,36 @@ def serving(args):
                              "p99_s": round(pct(seq_lat, 99), 4),
                              "makespan_s": round(seq_span, 3)},
         "throughput_ratio": round((len(res) / span) / (len(reqs) / seq_span), 3),
+        "p50_ratio_sequential_over_batched": round(pct(seq_lat, 50) / max(pct(lat, 50), 1e-9), 3),
         "energies_bit_identical": f"{same}/{len(reqs)}",
     }
+
+
+def serving(args):
+    """Poisson streams of independent solves (norb 16-40, n 4-10, 200-8000 strings
+    per spin, per-request integrals), served with continuous batching, against the
+    same stream solved one request at a time with the plain solver.  Two loads:
+    "overload" (arrivals far faster than the GPU completes solves: throughput)
+    and "steady" (about half the sequential service rate: latency, where a
+    small solve that arrives behind a large one waits for it when solves run
+    one at a time)."""
+    import sbd_amd
+    from sbd_amd import synth
+    opt = sbd_amd.SBD(max_it=args.serve_max_it, max_nb=10, eps=args.serve_eps, init=0)
+    out = {"options": {"max_it": args.serve_max_it, "max_nb": 10, "eps": args.serve_eps,
+                       "init": 0}}
+    for label, count, rate, seed in (("overload", args.serve_count, args.serve_rate, 4004),
+                                     ("steady", args.serve_steady_count, args.serve_steady_rate,
+                                      4005)):
+        if count <= 0:
+            continue
+        t0 = time.time()
+        reqs = synth.serving_requests(count, seed, rate=rate)
+        log(f"serving {label}: {len(reqs)} requests at {rate}/s generated in "
+            f"{time.time() - t0:.1f}s, N total {sum(len(r['a']) * len(r['b']) for r in reqs):.3e}")
+        out[label] = serve_stream(reqs, opt, args, label)
+        out[label]["arrival_rate_per_s"] = rate
+    return out
 
 
 def main():
@@ -176,9 +197,15 @@ def main():
     ap.add_argument("--dav-restarts", type=int, default=60,
                     help="Davidson max_it (restarts of max_nb=10 steps each)")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-rdm", action="store_true")
     ap.add_argument("--serve-count", type=int, default=64,
                     help="requests in the stream (the serving configuration names 256)")
-    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=40.0,
+                    help="Poisson arrivals per second of the overload stream")
+    ap.add_argument("--serve-steady-count", type=int, default=32,
+                    help="requests in the steady (sub-saturation) stream; 0 skips it")
+    ap.add_argument("--serve-steady-rate", type=float, default=0.5,
+                    help="Poisson arrivals per second of the steady stream")
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-mem-gb", type=int, default=160)
     ap.add_argument("--serve-eps", type=float, default=1e-6)
@@ -316,6 +343,24 @@ def main():
             f"(bit-exact rows: {out['cpu_baseline']['sample_bitexact_vs_gpu']})")
     else:
         out["cpu_baseline"] = None
+    if not args.no_rdm:
+        # full one- and two-particle RDMs (do_rdm != 0) of the normalised input
+        # vector: wall time of one sbd_correlation call (device vector in, host
+        # matrices out), median of three
+        x = T / torch.linalg.norm(T)
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            one, two = m.correlation(x)
+            times.append(time.perf_counter() - t0)
+        L = w["norb"]
+        tr = [float(np.trace(one[s].reshape(L, L))) for s in range(2)]
+        out["rdm"] = {"ms_per_call": round(1e3 * sorted(times)[1], 2), "calls": 3,
+                      "trace_alpha": tr[0], "trace_beta": tr[1],
+                      "outputs": "onebody 2 L^2 and twobody 4 L^4 doubles on the host"}
+        log(f"rdm: {out['rdm']['ms_per_call']:.1f} ms per call; traces {tr[0]:.12f} {tr[1]:.12f}")
+        del x, one, two
     m.close()
     del T, Wb, hii
     if not args.no_serving:

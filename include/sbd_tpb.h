@@ -31,6 +31,8 @@
  *                                                        sbd_occupation
  *   sbd::CarryOverAdet / CarryOverBdet chemistry/tpb/rdmat.h:60-134 (+ beta)
  *                                                        sbd_carryover
+ *   sbd::Correlation     chemistry/tpb/correlation.h:379-643 (do_rdm != 0;
+ *                        device form correlation_thrust.h) sbd_correlation
  *   sbd::SaveWavefunction / LoadWavefunction chemistry/tpb/restart.h:20-65,
  *                        414-657                         sbd_save_wavefunction,
  *                                                        sbd_load_wavefunction
@@ -250,6 +252,21 @@ int sbd_string_weights(sbd_ctx* ctx, const double* W, double* wa, double* wb, vo
  * orbital): density[2o] = alpha and density[2o+1] = beta occupation of
  * orbital o (host array of 2 norb doubles), the reference's `density` output. */
 int sbd_occupation(sbd_ctx* ctx, const double* W, double* density, void* stream);
+/* One- and two-particle reduced density matrices (Correlation,
+ * tpb/correlation.h:379-643, with ZeroDiff/OneDiff/TwoDiffCorrelation of
+ * basic/correlation.h:16-247; tpb/sbdiag.h:704-728 for do_rdm != 0).  Host
+ * outputs in the reference's layout:
+ *   onebody[s*L^2 + o1 + L*o2]          (2 L^2 doubles; s = 0 alpha, 1 beta)
+ *   twobody[(s1 + 2*s2)*L^4 + o1 + L*o2 + L^2*o3 + L^3*o4]   (4 L^4 doubles)
+ * with L = norb; same-spin blocks equal <a+_{o1} a+_{o2} a_{o4} a_{o3}>.  The
+ * alpha x beta double-excitation terms are written where the reference writes
+ * them, including the terms whose lowest bra-only spin orbital has the spin
+ * of the highest ket-only one, which the reference adds with the block index
+ * s1 + 2*s2 of the other spin assignment.  Sums are taken in a fixed order;
+ * the reference's order (atomics) differs, so results agree with it to
+ * rounding.  Requires sbd_make_helpers; uses 8 N bytes of device scratch. */
+int sbd_correlation(sbd_ctx* ctx, const double* W, double* onebody, double* twobody,
+                    void* stream);
 /* Carry-over strings (CarryOverAdet / CarryOverBdet, tpb/rdmat.h:60-134; the
  * selection of tpb/sbdiag.h:745-772): strings in descending order of weight
  * (ties by index).  ratio == 0: keep up to and including the first string

@@ -11,7 +11,10 @@ product path never imports it.
                    (bit_manipulation.h:421-449; ascending value, unique)
 * setup_integrals: chemistry/basic/makeintegrals.h:124-174 (in C)
 * System         : tables (tpb/helper.h:185-342), hii (tpb/qcham.h:290-340),
-                   sigma (tpb/mult.h:625-881), Davidson (tpb/davidson.h:707-970)
+                   sigma (tpb/mult.h:625-881), Davidson (tpb/davidson.h:707-970),
+                   Lanczos (tpb/lanczos.h:162-327), weights / occupations /
+                   carry-over (tpb/rdmat.h, tpb/occupation.h), RDMs
+                   (tpb/correlation.h:379-643)
 """
 from __future__ import annotations
 
@@ -59,6 +62,7 @@ def lib():
         L.orc_occupation.argtypes = [C.c_void_p, dp, dp]
         L.orc_carryover.restype = C.c_long
         L.orc_carryover.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, lp, dp]
+        L.orc_correlation.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_hdiag_rows.argtypes = [C.c_void_p, dp, C.c_long, C.c_long]
         L.orc_sigma_rows.argtypes = [C.c_void_p, dp, dp, dp, C.c_long, C.c_long]
         L.orc_det_from_ab.argtypes = [up, up, C.c_long, C.c_int, up]
@@ -272,6 +276,17 @@ class System:
         m = lib().orc_carryover(self._h, _p(W, C.c_double), spin, threshold, ratio,
                                 out.ctypes.data_as(C.POINTER(C.c_long)), C.byref(tr))
         return out[:m], tr.value
+
+    def correlation(self, W):
+        """correlation.h:379-643 (do_rdm != 0): (onebody (2, L, L), twobody
+        (4, L, L, L, L)); element [s][o1 + L*o2] is onebody[s, o2, o1] and
+        [s1 + 2*s2][o1 + L*o2 + L^2*o3 + L^3*o4] is twobody[s1 + 2*s2, o4, o3, o2, o1]."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        L = self.L
+        one, two = np.zeros(2 * L * L), np.zeros(4 * L ** 4)
+        lib().orc_correlation(self._h, _p(W, C.c_double), _p(one, C.c_double),
+                              _p(two, C.c_double))
+        return one.reshape(2, L, L), two.reshape(4, L, L, L, L)
 
     def energy(self, hii, W):
         """sbdiag.h:488-526: E = <W, H W> with a fresh sigma."""

@@ -3,7 +3,8 @@
  *
  * CPU restatement of the reference's tensor-product-basis (TPB) path: excitation
  * tables, diagonal, sigma (H*v), single-root Davidson and Lanczos, and the
- * measurements on a solved vector (string weights, occupations, carry-over).  It is the parity
+ * measurements on a solved vector (string weights, occupations, carry-over,
+ * the full one- and two-particle RDMs).  It is the parity
  * checker for the HIP library and the timed CPU baseline; it is never part of
  * the product path.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline leg may load it.
@@ -17,8 +18,9 @@
  * bit_length.
  *
  * Build: see oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
- * Parity pins: see tests/test_oracle_*.py (golden vectors from the reference's
- * own tests, reference-held energies, brute-force second quantisation).
+ * Parity pins: tests/test_oracle.py and tests/test_host_abi.py (golden vectors
+ * from the reference's own tests, reference-held energies, brute-force second
+ * quantisation: the Hamiltonian matrix and the RDMs, closed forms).
  */
 #include <math.h>
 #include <omp.h>
@@ -774,4 +776,132 @@ void orc_sigma_rows(const orc_sys *S, const double *hii, const double *T, double
       }
     }
   }
+}
+
+/* ------------------------------------------------------------------ */
+/*
+ * Correlation (do_rdm != 0), chemistry/tpb/correlation.h:379-643 at one rank,
+ * with ZeroDiffCorrelation, OneDiffCorrelation and TwoDiffCorrelation of
+ * chemistry/basic/correlation.h:16-50, 103-148, 190-247.  one[s*L*L + o1 + L*o2]
+ * (s = 0 alpha, 1 beta) and two[(s1 + 2*s2)*L^4 + o1 + L*o2 + L^2*o3 + L^3*o4]
+ * are written as the reference writes onebody[s] and twobody[s1 + 2*s2]:
+ *   diagonal: every (ia, ib);
+ *   task type 2: alpha singles and doubles (same beta string);
+ *   task type 1: beta singles and doubles (same alpha string);
+ *   task type 0: an alpha single times a beta single.
+ * Each pair (bra I, ket J) adds W_I * W_J times the parity of the interleaved
+ * bra determinant, in the loop order of the reference's one-thread run.
+ */
+static void zero_diff(const uint64_t *D, double w, int L, double *one, double *two) {
+  const long L2 = (long)L * L, L3 = L2 * L, L4 = L3 * L;
+  const double w2 = w * w;
+  for (int i = 0; i < 2 * L; i++) {
+    if (!getocc(D, i)) continue;
+    const int oi = i / 2, si = i % 2;
+    one[si * L2 + oi + (long)L * oi] += w2;
+    for (int j = i + 1; j < 2 * L; j++) {
+      if (!getocc(D, j)) continue;
+      const int oj = j / 2, sj = j % 2;
+      two[(si + 2 * sj) * L4 + oi + L * oj + L2 * oi + L3 * oj] += w2;
+      two[(sj + 2 * si) * L4 + oj + L * oi + L2 * oj + L3 * oi] += w2;
+      if (si == sj) {
+        two[(si + 2 * sj) * L4 + oi + L * oj + L2 * oj + L3 * oi] += -w2;
+        two[(sj + 2 * si) * L4 + oj + L * oi + L2 * oi + L3 * oj] += -w2;
+      }
+    }
+  }
+}
+
+static void one_diff(const uint64_t *D, double wi, double wj, int L, int i, int a, double *one,
+                     double *two) {
+  const long L2 = (long)L * L, L3 = L2 * L, L4 = L3 * L;
+  double sgn = 1.0;
+  parity(D, i < a ? i : a, i < a ? a : i, &sgn);
+  const int oi = i / 2, si = i % 2, oa = a / 2;
+  const double v = wi * wj * sgn;
+  one[si * L2 + oi + (long)L * oa] += v;
+  for (int x = 0; x < 2; x++) {
+    uint64_t bits = D[x];
+    while (bits) {
+      const int pos = __builtin_ctzll(bits);
+      const int soj = x * 64 + pos, oj = soj / 2, sj = soj % 2;
+      two[(si + 2 * sj) * L4 + oa + oj * L + oi * L2 + oj * L3] += v;
+      two[(sj + 2 * si) * L4 + oj + oa * L + oj * L2 + oi * L3] += v;
+      if (si == sj) {
+        two[(si + 2 * sj) * L4 + oa + oj * L + oj * L2 + oi * L3] += -v;
+        two[(sj + 2 * si) * L4 + oj + oa * L + oi * L2 + oj * L3] += -v;
+      }
+      bits &= bits - 1;
+    }
+  }
+}
+
+static void two_diff(const uint64_t *D, double wi, double wj, int L, int i, int j, int a, int b,
+                     double *two) {
+  const long L2 = (long)L * L, L4 = L2 * L2;
+  double sgn = 1.0;
+  const int I = i < j ? i : j, J = i < j ? j : i;
+  const int A = a < b ? a : b, B = a < b ? b : a;
+  parity(D, I < A ? I : A, I < A ? A : I, &sgn);
+  parity(D, J < B ? J : B, J < B ? B : J, &sgn);
+  if (A > J || B < I) sgn *= -1.0;
+  const int oi = I / 2, si = I % 2, oa = A / 2, sa = A % 2;
+  const int oj = J / 2, sj = J % 2, ob = B / 2, sb = B % 2;
+  const double v = sgn * wi * wj;
+  if (si == sa) {
+    two[(si + 2 * sj) * L4 + oa + L * ob + L2 * (oi + L * oj)] += v;
+    two[(sj + 2 * si) * L4 + ob + L * oa + L2 * (oj + L * oi)] += v;
+  }
+  if (si == sb) {
+    two[(si + 2 * sj) * L4 + oa + L * ob + L2 * (oj + L * oi)] += -v;
+    two[(sj + 2 * si) * L4 + ob + L * oa + L2 * (oi + L * oj)] += -v;
+  }
+}
+
+void orc_correlation(const orc_sys *S, const double *W, double *one, double *two) {
+  const int L = S->L;
+  const long na = S->na, nb = S->nb, L2 = (long)L * L;
+  memset(one, 0, 2 * L2 * sizeof(double));
+  memset(two, 0, 4 * L2 * L2 * sizeof(double));
+  uint64_t D[2];
+  for (long ia = 0; ia < na; ia++)
+    for (long ib = 0; ib < nb; ib++) {
+      det_from_ab(S->a[ia], S->b[ib], L, D);
+      zero_diff(D, W[ia * nb + ib], L, one, two);
+    }
+  /* task type 2: alpha singles, then alpha doubles, per bra (ia, ib) */
+  for (long ia = 0; ia < na; ia++)
+    for (long ib = 0; ib < nb; ib++) {
+      det_from_ab(S->a[ia], S->b[ib], L, D);
+      const double wi = W[ia * nb + ib];
+      for (long k = S->as_off[ia]; k < S->as_off[ia + 1]; k++)
+        one_diff(D, wi, W[S->as_ket[k] * nb + ib], L, S->as_cr[2 * k], S->as_cr[2 * k + 1], one,
+                 two);
+      for (long k = S->ad_off[ia]; k < S->ad_off[ia + 1]; k++)
+        two_diff(D, wi, W[S->ad_ket[k] * nb + ib], L, S->ad_cr[4 * k], S->ad_cr[4 * k + 1],
+                 S->ad_cr[4 * k + 2], S->ad_cr[4 * k + 3], two);
+    }
+  /* task type 1: beta singles, then beta doubles */
+  for (long ia = 0; ia < na; ia++)
+    for (long ib = 0; ib < nb; ib++) {
+      det_from_ab(S->a[ia], S->b[ib], L, D);
+      const double wi = W[ia * nb + ib];
+      for (long k = S->bs_off[ib]; k < S->bs_off[ib + 1]; k++)
+        one_diff(D, wi, W[ia * nb + S->bs_ket[k]], L, S->bs_cr[2 * k], S->bs_cr[2 * k + 1], one,
+                 two);
+      for (long k = S->bd_off[ib]; k < S->bd_off[ib + 1]; k++)
+        two_diff(D, wi, W[ia * nb + S->bd_ket[k]], L, S->bd_cr[4 * k], S->bd_cr[4 * k + 1],
+                 S->bd_cr[4 * k + 2], S->bd_cr[4 * k + 3], two);
+    }
+  /* task type 0: alpha single x beta single, TwoDiffCorrelation(i = alpha cr,
+   * j = beta cr, a = alpha an, b = beta an) */
+  for (long ia = 0; ia < na; ia++)
+    for (long ib = 0; ib < nb; ib++) {
+      det_from_ab(S->a[ia], S->b[ib], L, D);
+      const double wi = W[ia * nb + ib];
+      for (long k = S->as_off[ia]; k < S->as_off[ia + 1]; k++)
+        for (long m = S->bs_off[ib]; m < S->bs_off[ib + 1]; m++)
+          two_diff(D, wi, W[S->as_ket[k] * nb + S->bs_ket[m]], L, S->as_cr[2 * k],
+                   S->bs_cr[2 * m], S->as_cr[2 * k + 1], S->bs_cr[2 * m + 1], two);
+    }
 }

@@ -32,7 +32,8 @@ EXPORTS = [
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
     "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
-    "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_carryover", "sbd_diag_from",
+    "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_correlation", "sbd_carryover",
+    "sbd_diag_from",
     "sbd_save_wavefunction", "sbd_load_wavefunction", "sbd_singles_extend",
     "sbd_singles_extend_strings", "sbd_sort_strings", "sbd_decode_strings_bin",
     "sbd_encode_strings_bin",
@@ -126,6 +127,7 @@ def lib():
     L.sbd_save_wavefunction.argtypes = [vp, C.c_char_p, C.c_size_t, dp]
     L.sbd_load_wavefunction.argtypes = [vp, C.c_char_p, C.c_size_t, dp, up]
     L.sbd_occupation.argtypes = [vp, vp, dp, vp]
+    L.sbd_correlation.argtypes = [vp, vp, dp, dp, vp]
     L.sbd_carryover.argtypes = [vp, vp, C.c_double, C.c_double, up, szp, up, szp, dp, vp]
     L.sbd_diag.argtypes = [vp, C.POINTER(Options), dp, dp, dp, C.c_int, C.POINTER(Report)]
     L.sbd_diag_footprint.argtypes = [vp, C.POINTER(Options), up]
@@ -171,7 +173,7 @@ class SBD:
     init: int = 0
     seed: int = 1729
     bit_length: int = 20
-    do_rdm: int = 0              # 0: occupation density only (full RDMs not built)
+    do_rdm: int = 0              # 0: occupation density; else also the full 1- and 2-RDMs
     carryover_type: int = 0      # 0 none, 1 dominant strings, 2 + singles, 3 singles of W^2 > threshold
     ratio: float = 0.0
     threshold: float = 0.01
@@ -497,6 +499,17 @@ class MultTPB:
                self._ctx)
         return out
 
+    def correlation(self, W):
+        """Correlation (correlation.h:379-643, do_rdm != 0): (onebody, twobody) as
+        the reference's one_p_rdm / two_p_rdm, arrays of shape (2, L*L) and
+        (4, L**4): onebody[s][o1 + L*o2], twobody[s1 + 2*s2][o1 + L*o2 + L^2*o3 + L^3*o4]."""
+        L = self.norb
+        one = np.zeros((2, L * L))
+        two = np.zeros((4, L ** 4))
+        _check(lib().sbd_correlation(self._ctx, _dptr(W), _ptr(one, C.c_double),
+                                     _ptr(two, C.c_double), _stream()), self._ctx)
+        return one, two
+
     def carryover(self, W, threshold=0.01, ratio=0.0):
         """CarryOverAdet/Bdet (rdmat.h:60-134): (alpha strings, beta strings,
         truncated weights) in descending order of weight."""
@@ -533,22 +546,23 @@ class MultTPB:
 class DiagResult:
     """Outputs of sbd::tpb::diag (chemistry/tpb/sbdiag.h:151-163): energy,
     occupation density (density[2o] alpha, [2o+1] beta), carry-over strings
-    (empty unless carryover_type is 1) and the final vector."""
+    (empty unless carryover_type is 1..3), the final vector and, for do_rdm != 0,
+    the one- and two-particle RDMs (empty otherwise)."""
     energy: float
     density: np.ndarray
     co_adet: np.ndarray
     co_bdet: np.ndarray
     W: np.ndarray
+    one_p_rdm: np.ndarray = None
+    two_p_rdm: np.ndarray = None
 
 
 def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
     """sbd::tpb::diag (one rank; method 0/1 Davidson, 2/3 Lanczos), followed by
-    the occupation density and, with carryover_type 1, the carry-over strings
-    (sbdiag.h:666-777).  Returns a DiagResult."""
+    the occupation density (do_rdm == 0) or the full one- and two-particle RDMs
+    with the density read off the one-particle diagonal (do_rdm != 0), and the
+    carry-over strings (sbdiag.h:666-791).  Returns a DiagResult."""
     import torch
-    if sbd_data.do_rdm != 0:
-        raise SbdError("do_rdm != 0 (full one- and two-particle RDMs) is not supported by this "
-                       "build; do_rdm = 0 gives the occupation density")
     if sbd_data.carryover_type not in (0, 1, 2, 3):
         raise SbdError("carryover_type must be 0..3")
     m = MultTPB(fcidump.norb, adet, bdet, fcidump=fcidump)
@@ -563,7 +577,17 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
         if savename:
             m.save_wavefunction(savename, W, sbd_data.bit_length)
         Wd = torch.tensor(W, device="cuda")
-        density = m.occupation(Wd)
+        one = two = np.zeros((0, 0))
+        if sbd_data.do_rdm == 0:
+            density = m.occupation(Wd)
+        else:
+            # sbdiag.h:704-728: density[2o + s] = one_p_rdm[s][o + L*o]
+            one, two = m.correlation(Wd)
+            L = m.norb
+            density = np.zeros(2 * L)
+            for o in range(L):
+                density[2 * o] = one[0][o + L * o]
+                density[2 * o + 1] = one[1][o + L * o]
         co_a = np.zeros(0, dtype=np.uint64)
         co_b = np.zeros(0, dtype=np.uint64)
         # carry-over strings (sbdiag.h:745-791): 1 dominant strings, 2 those
@@ -579,7 +603,7 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
         del Wd
     finally:
         m.close()
-    return DiagResult(E, density, co_a, co_b, W)
+    return DiagResult(E, density, co_a, co_b, W, one, two)
 
 
 class Server:

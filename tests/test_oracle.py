@@ -307,3 +307,56 @@ def test_golden_energies_consistent(O):
     for key, d in g.items():
         assert d["residual"] < 1e-8
         assert d["energy"] < d["hii_min"]
+
+
+@pytest.mark.parametrize("L,na,nb,seed,sub", [(4, 2, 2, 3, 0), (5, 2, 2, 9, 0), (5, 3, 1, 4, 0),
+                                              (6, 3, 2, 5, 12)])
+def test_oracle_correlation_matches_second_quantisation(O, L, na, nb, seed, sub):
+    """Correlation (tpb/correlation.h:379-643) against RDMs built from second
+    quantisation (oracle/brute.py): the one-particle RDM and the same-spin
+    two-particle blocks equal <a+ a> and <a+_p a+_q a_t a_r> exactly; the alpha x
+    beta blocks do too, except that the reference adds the alpha x beta double
+    excitations whose lowest bra-only spin orbital has the spin of the highest
+    ket-only one into the other alpha x beta block (index s1 + 2 s2 with the
+    spins exchanged, TwoDiffCorrelation's si == sb branch).  Those terms are
+    large here, so a formula that put them in the textbook block fails."""
+    from oracle import brute
+    from sbd_amd import synth
+    a, b = synth.all_strings(L, na), synth.all_strings(L, nb)
+    if sub:
+        rng = np.random.default_rng(seed)
+        a = np.sort(rng.choice(a, sub, replace=False))
+        b = np.sort(rng.choice(b, min(sub, len(b)), replace=False))
+    _, h, eri = synth.make_integrals(L, seed, 2 * L)
+    S = O.System.from_workload(dict(norb=L, I0=0.0, h=h, eri=eri, a=a, b=b))
+    W = np.random.default_rng(seed).standard_normal(S.N)
+    W /= np.linalg.norm(W)
+    one, two = S.correlation(W)
+    ob, tb, swap = brute.rdm(L, a, b, W)
+    assert np.abs(one - ob).max() < 1e-14
+    assert np.abs(two[[0, 3]] - tb[[0, 3]]).max() < 1e-14
+    expected = tb - swap + swap[[0, 2, 1, 3]]
+    assert np.abs(two - expected).max() < 1e-14
+    if nb > 1 and na > 1:
+        assert np.abs(swap).max() > 1e-2
+    # traces: sum_p <n_p> per spin = electrons of that spin
+    assert abs(np.trace(one[0]) - na) < 1e-13 and abs(np.trace(one[1]) - nb) < 1e-13
+
+
+@pytest.mark.parametrize("name,scale", [("fci12", 0.2), ("sci16open", 1.0)])
+def test_oracle_correlation_energy_identity(O, name, scale):
+    """I0 + sum_pq h_pq <a+_p a_q> + 1/2 sum (pr|qt) Gamma[p,q,r,t] over the four
+    spin blocks equals <W|H|W> from the sigma: every two-body matrix element is
+    weighted once (the reference's relocated alpha x beta terms stay inside the
+    alpha x beta sum)."""
+    from sbd_amd import synth
+    w = synth.make_workload(name, scale)
+    S = O.System.from_workload(w)
+    W = synth.random_init_vector(S.N, 5)
+    W /= np.linalg.norm(W)
+    one, two = S.correlation(W)
+    L = w["norb"]
+    eri = synth.unpack_eri(w["eri"], L)
+    E = w["I0"] + sum(np.einsum("qp,pq->", one[s], w["h"]) for s in range(2))
+    E += 0.5 * sum(np.einsum("trqp,prqt->", two[k], eri) for k in range(4))
+    assert abs(E - S.energy(S.hdiag(), W)) < 1e-12 * abs(E)
