@@ -1,8 +1,8 @@
 """GPU parity of the full one- and two-particle RDMs (do_rdm != 0; Correlation,
 chemistry/tpb/correlation.h:379-643) through the C ABI against the oracle.
 
-* small bases (16 and 34 orbitals, closed and open shell, dense and sparse
-  connections): every entry of both matrices within 1e-12 of the oracle (the
+* small bases (10 to 48 orbitals, closed and open shell, dense and sparse
+  connections, a single beta string): every entry of both matrices within 1e-12 of the oracle (the
   sums are ordered differently, as the reference's atomics order them
   differently from run to run);
 * larger bases, where the oracle would take minutes: size-independent
@@ -45,6 +45,16 @@ def _workload(name):
         I0, h, eri = synth.make_integrals(34, 4242, 68)
         return dict(norb=34, I0=I0, h=h, eri=eri, a=synth.random_strings(34, 3, 300, 11),
                     b=synth.random_strings(34, 2, 200, 12))
+    if name == "l48":
+        # 48 orbitals: spectator masks reach past bit 40 (unpacked tables)
+        I0, h, eri = synth.make_integrals(48, 4848, 96)
+        return dict(norb=48, I0=I0, h=h, eri=eri, a=synth.random_strings(48, 2, 400, 13),
+                    b=synth.random_strings(48, 2, 300, 14))
+    if name == "one_beta":
+        # a single beta string: no beta excitations, no alpha x beta terms
+        I0, h, eri = synth.make_integrals(10, 1010, 20)
+        return dict(norb=10, I0=I0, h=h, eri=eri, a=synth.all_strings(10, 3),
+                    b=np.array([0b0000000111], dtype=np.uint64))
     scale = {"fci12": 0.2, "sci16": 1.0, "sci16open": 1.0}[name]
     return synth.make_workload(name, scale)
 
@@ -59,7 +69,7 @@ def _rdm_energy(w, one, two):
     return E + 0.5 * sum(np.einsum("trqp,prqt->", t[k], eri) for k in range(4))
 
 
-@pytest.mark.parametrize("name", ["fci12", "sci16", "sci16open", "l34"])
+@pytest.mark.parametrize("name", ["fci12", "sci16", "sci16open", "l34", "l48", "one_beta"])
 def test_correlation_matches_oracle(torch, name):
     import sbd_amd
     from oracle import oracle as O
@@ -68,7 +78,8 @@ def test_correlation_matches_oracle(torch, name):
     w = _workload(name)
     ref = O.System.from_workload(w)
     m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
-    assert ref.counts()[0] > 0 and ref.counts()[2] > 0, "needs single excitations of both spins"
+    if name != "one_beta":
+        assert ref.counts()[0] > 0 and ref.counts()[2] > 0, "needs singles of both spins"
     W = synth.random_init_vector(m.N, 17)
     W /= np.linalg.norm(W)
     one, two = m.correlation(torch.tensor(W, device="cuda"))
@@ -78,6 +89,8 @@ def test_correlation_matches_oracle(torch, name):
     assert np.abs(two - r2.reshape(4, L ** 4)).max() < 1e-12
     # the alpha x beta blocks are populated (the oracle pins their layout)
     assert np.abs(r2[1]).max() > 1e-6 and np.abs(r2[2]).max() > 1e-6
+    n_alpha = bin(int(w["a"][0])).count("1")
+    assert abs(np.trace(one[0].reshape(L, L)) - n_alpha) < 1e-12
     m.close()
 
 
