@@ -306,8 +306,7 @@ def main():
     if not args.no_davidson:
         res = {}
         for label, prec in (("fp64", 0), ("mixed", 1)):
-            opt = sbd_amd.SBD(max_it=args.dav_restarts, max_nb=10, eps=1e-7, precision=prec,
-                              fp32_eps=1e-4)
+            opt = sbd_amd.SBD(max_it=args.dav_restarts, max_nb=10, eps=1e-7, precision=prec)
             t0 = time.time()
             E, _, rep, hist = m.diag(opt, history=True)
             res[label] = {"energy": E, "steps": rep["steps"], "steps_fp32": rep["steps_fp32"],

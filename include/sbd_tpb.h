@@ -98,7 +98,7 @@ typedef struct {
   int precision;     /* 0 = FP64; 1 = FP32 vectors/sigma + FP64 reductions,
                         Ritz and Gram-Schmidt, then an FP64 correction sweep   */
   double fp32_eps;   /* precision = 1: residual at which FP32 hands over to FP64
-                        (default 1e-4)                                         */
+                        (default 1e-2)                                         */
   int verbose;       /* 1 = print per-step lines like the reference            */
 } sbd_options;
 

@@ -178,7 +178,7 @@ class SBD:
     ratio: float = 0.0
     threshold: float = 0.01
     precision: int = 0           # added: 1 = FP32 vectors + FP64 correction
-    fp32_eps: float = 1.0e-4     # added: FP32 -> FP64 hand-over residual
+    fp32_eps: float = 1.0e-2     # added: FP32 -> FP64 hand-over residual
     verbose: int = 0
 
     def options(self) -> Options:

@@ -392,7 +392,7 @@ void sbd_options_default(sbd_options* o) {
   o->init = 0;
   o->seed = 1729;
   o->precision = 0;
-  o->fp32_eps = 1.0e-4;
+  o->fp32_eps = 1.0e-2;
   o->verbose = 0;
 }
 
