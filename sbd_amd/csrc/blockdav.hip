@@ -43,19 +43,13 @@ struct Buf {
   }
 };
 
-// x[i] = 2 u_i - 1, u_i the i-th output of the splitmix64 stream seeded with
-// seed (the stream sbd_diag's init = 1 uses, counter-based so each element is
-// computed on its own)
+// x[i] = uniform_pm1(elem_hash(seed, i)): the elements of the seeded random
+// vector of sbd_diag's init = 1 (before normalisation), each computed on its own
 template <typename VT>
 __global__ void k_random_vec(VT* __restrict__ x, uint64_t n, uint64_t seed) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    z ^= z >> 31;
-    x[i] = (VT)(2.0 * ((z >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
-  }
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = (VT)uniform_pm1(elem_hash(seed, i));
 }
 
 template <typename VT>

@@ -83,6 +83,28 @@ int set_err(sbd_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+// Randomize, framework/dm_vector.h:101-191 at one rank and one thread (the
+// reference's per-thread compensated sums added in thread order reduce to one
+// compensated sum in index order).
+void randomize(double* W, uint64_t N, uint64_t seed) {
+  if (seed == 0) {
+    const uint64_t ns = (uint64_t)std::chrono::high_resolution_clock::now()
+                            .time_since_epoch()
+                            .count();
+    seed = elem_hash(ns, 0);  // sbd_time_seed at rank 0
+  }
+  double sum = 0.0, eps = 0.0;
+  for (uint64_t i = 0; i < N; i++) {
+    W[i] = uniform_pm1(elem_hash(seed, i));
+    const double val = W[i] * W[i] - eps;
+    const double tmp = sum + val;
+    eps = (tmp - sum) - val;
+    sum = tmp;
+  }
+  const double factor = 1.0 / std::sqrt(sum);
+  for (uint64_t i = 0; i < N; i++) W[i] *= factor;
+}
+
 }  // namespace sbdx
 
 using namespace sbdx;
@@ -732,21 +754,7 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
   } else if (opt->init == 0) {
     W0[0] = 1.0;
   } else {
-    // Randomize (framework/dm_vector.h:101-191 semantics: uniform(-1,1), then
-    // normalised); the stream here is splitmix64 seeded with opt->seed.
-    uint64_t s = opt->seed;
-    double nrm = 0.0;
-    for (uint64_t i = 0; i < N; i++) {
-      s += 0x9E3779B97F4A7C15ULL;
-      uint64_t z = s;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-      z ^= z >> 31;
-      W0[i] = 2.0 * ((z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
-      nrm += W0[i] * W0[i];
-    }
-    nrm = 1.0 / std::sqrt(nrm);
-    for (auto& x : W0) x *= nrm;
+    randomize(W0.data(), N, opt->seed);
   }
   DevBuf<double> hii, W, HW;
   if (hipMalloc(&hii.p, 8 * N) != hipSuccess || hipMalloc(&W.p, 8 * N) != hipSuccess)

@@ -80,6 +80,23 @@ __host__ __device__ inline uint32_t alpha_par_double(uint64_t PA, uint32_t c) {
                     (PA >> code_f(c, 3))) & 1u;
 }
 
+// Seeded random vector elements (Randomize, framework/dm_vector.h:101-191):
+// element i of a vector drawn with seed s is uniform_pm1(elem_hash(s, i)), so
+// any element is computed on its own and the vector is the same for any split
+// of the work.  elem_hash mixes (seed, index) linearly and applies the
+// splitmix64 output finaliser; uniform_pm1 maps the top 53 bits to [-1, 1).
+__host__ __device__ inline uint64_t elem_hash(uint64_t seed, uint64_t idx) {
+  uint64_t x = seed * 6364136223846793005ULL + idx * 2862933555777941757ULL;
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
+}
+__host__ __device__ inline double uniform_pm1(uint64_t h) {
+  return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
 __host__ __device__ inline uint64_t spread32(uint64_t v) {
   v &= 0xFFFFFFFFULL;
   v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
@@ -260,6 +277,11 @@ int vec_convert(sbd_ctx* ctx, const VS* x, VD* y, uint64_t n, hipStream_t st);
 // returns code.
 double now_s();
 int set_err(sbd_ctx* ctx, int code, const std::string& msg);
+// W[0, N) = the normalised seeded random vector of BasisInitVector's init = 1
+// (Randomize: elements from elem_hash, squared norm summed with Kahan
+// compensation in index order, then scaled by 1 / norm); seed 0 draws a seed
+// from the clock, as the reference does.
+void randomize(double* W, uint64_t N, uint64_t seed);
 
 // Restarted Lanczos (chemistry/tpb/lanczos.h:162-327) on device vectors; W is
 // the start vector on entry and sum_ib U[ib, 0] C[ib] of the last pass on exit.

@@ -217,19 +217,7 @@ struct sbd_server {
     if (q.opt.init == 0) {
       W0[0] = 1.0;
     } else {
-      uint64_t sd = q.opt.seed;
-      double nrm = 0.0;
-      for (uint64_t i = 0; i < N; i++) {
-        sd += 0x9E3779B97F4A7C15ULL;
-        uint64_t z = sd;
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-        z ^= z >> 31;
-        W0[i] = 2.0 * ((z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
-        nrm += W0[i] * W0[i];
-      }
-      nrm = 1.0 / std::sqrt(nrm);
-      for (auto& x : W0) x *= nrm;
+      randomize(W0.data(), N, q.opt.seed);
     }
     if (hipMemcpy(s->C[0], W0.data(), 8 * N, hipMemcpyHostToDevice) != hipSuccess)
       return SBD_ERR_DEVICE;

@@ -59,9 +59,39 @@ class SplitMix64:
         return z[:n]
 
 
+def elem_hash(seed: int, idx: np.ndarray) -> np.ndarray:
+    """Per-element hash of the reference's Randomize (framework/dm_vector.h:102-113):
+    a linear mix of (seed, index) followed by the splitmix64 output finaliser."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = np.uint64(seed) * np.uint64(6364136223846793005) + idx * np.uint64(2862933555777941757)
+        x ^= x >> np.uint64(30)
+        x *= np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(27)
+        x *= np.uint64(0x94D049BB133111EB)
+        x ^= x >> np.uint64(31)
+    return x
+
+
+def randomize(N: int, seed: int) -> np.ndarray:
+    """The init = 1 start vector (BasisInitVector -> Randomize, davidson.h:178-181,
+    dm_vector.h:114-191) at one rank and one thread: element i = the top 53 bits
+    of elem_hash(seed, i) mapped to [-1, 1), scaled by 1/sqrt of the squared norm
+    summed with Kahan compensation in index order (seed > 0)."""
+    h = elem_hash(seed, np.arange(N, dtype=np.uint64))
+    w = (h >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+    s = e = 0.0
+    for x in (w * w).tolist():
+        v = x - e
+        t = s + v
+        e = (t - s) - v
+        s = t
+    return w * (1.0 / math.sqrt(s))
+
+
 def random_init_vector(N: int, seed: int) -> np.ndarray:
-    """The initial vector of sbd_diag with init = 1: splitmix64(seed) uniforms
-    mapped to (-1, 1), normalised."""
+    """A seeded random test vector: splitmix64(seed) uniforms mapped to (-1, 1),
+    normalised (the init = 1 start vector is `randomize`)."""
     w = 2.0 * SplitMix64(seed).uniform(N) - 1.0
     return w / np.sqrt(np.dot(w, w))
 

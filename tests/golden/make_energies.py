@@ -1,7 +1,7 @@
 """Writes tests/golden/energies.json: oracle (CPU restatement) Davidson energies
 for the synthetic parity cases, converged to residual 1e-8 (energy error
 ~ residual^2, far below 1e-9 Ha).  The initial vector is the init = 1 vector
-(splitmix64 seed 1729, sbd_amd.synth.random_init_vector).  The cases are
+(the reference's Randomize with seed 1729, sbd_amd.synth.randomize).  The cases are
 densely connected bases: with the reference's diagonal preconditioner,
 Davidson stalls on nearly diagonal matrices (for a diagonal H the
 preconditioned residual is parallel to the current vector), which is what the
@@ -28,7 +28,7 @@ for key, (name, scale) in CASES.items():
     w = synth.make_workload(name, scale)
     S = O.System.from_workload(w)
     hii = S.hdiag()
-    W0 = synth.random_init_vector(S.N, SEED)
+    W0 = synth.randomize(S.N, SEED)
     W, hist = S.davidson(hii, W0, 100, 10, 1e-8)
     E = S.energy(hii, W)
     out[key] = {"workload": name, "scale": scale, "N": int(S.N), "energy": E, "init": 1,

@@ -141,6 +141,24 @@ def test_davidson_energy_vs_oracle(torch, name):
     m.close()
 
 
+def test_init1_start_vector_is_reference_randomize(torch):
+    """init = 1 (BasisInitVector, davidson.h:178-181 -> Randomize,
+    dm_vector.h:114-191): a one-vector solve that stops at its first step
+    returns its start vector, which equals the restatement synth.randomize."""
+    import sbd_amd
+    from sbd_amd import synth
+    say("init 1 start vector")
+    w = synth.make_workload("sci16", 1.0)
+    m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
+    for seed in (1729, 42):
+        E, W, rep = m.diag(sbd_amd.SBD(max_it=1, max_nb=1, eps=1e300, init=1, seed=seed),
+                           want_vector=True)
+        assert rep["steps"] == 1
+        W0 = synth.randomize(m.N, seed)
+        assert np.abs(W - W0).max() < 1e-15
+    m.close()
+
+
 @pytest.mark.parametrize("mol,alpha,E_known", [
     ("h2o", "h2o-1em3-alpha.txt.gz", -76.23594663),
     ("n2", "1em3-alpha.txt.gz", -109.04162110),

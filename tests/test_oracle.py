@@ -360,3 +360,32 @@ def test_oracle_correlation_energy_identity(O, name, scale):
     E = w["I0"] + sum(np.einsum("qp,pq->", one[s], w["h"]) for s in range(2))
     E += 0.5 * sum(np.einsum("trqp,prqt->", two[k], eri) for k in range(4))
     assert abs(E - S.energy(S.hdiag(), W)) < 1e-12 * abs(E)
+
+
+def test_randomize_is_pinned():
+    """The init = 1 start vector (BasisInitVector -> Randomize,
+    framework/dm_vector.h:102-191), restated in sbd_amd.synth.randomize:
+    * its hash ends in the splitmix64 output finaliser: with the (seed, index)
+      mix landing on the splitmix64 state k * 0x9E3779B97F4A7C15 it gives the
+      published first outputs of splitmix64 seeded with 0;
+    * elements depend on (seed, index) only: a shorter vector is a prefix of a
+      longer one up to the normalisation; the vector has unit norm and its
+      elements lie in [-1, 1) before scaling."""
+    from sbd_amd import synth
+    B = 2862933555777941757
+    inv = pow(B, -1, 1 << 64)
+    gamma = 0x9E3779B97F4A7C15
+    kat = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    for k, want in enumerate(kat, start=1):
+        idx = (k * gamma * inv) % (1 << 64)
+        assert int(synth.elem_hash(0, np.array([idx], dtype=np.uint64))[0]) == want
+    w_long = synth.randomize(5000, 1729)
+    w_short = synth.randomize(300, 1729)
+    assert abs(np.dot(w_long, w_long) - 1.0) < 1e-14
+    ratio = w_short / w_long[:300]
+    assert np.abs(ratio - ratio[0]).max() < 1e-13 * abs(ratio[0])
+    h = synth.elem_hash(1729, np.arange(5000, dtype=np.uint64))
+    u = (h >> np.uint64(11)).astype(np.float64) * (2.0 / 2.0 ** 53) - 1.0
+    assert u.min() >= -1.0 and u.max() < 1.0 and abs(u.mean()) < 0.05
+    assert np.abs(w_long - u / np.sqrt(np.dot(u, u))).max() < 1e-15
+    assert not np.array_equal(synth.randomize(300, 1730), w_short)
