@@ -197,7 +197,10 @@ class System:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_sys_free(self._h)
+            try:
+                lib().orc_sys_free(self._h)
+            except (TypeError, AttributeError):  # interpreter shutdown
+                pass
             self._h = None
 
     def counts(self):

@@ -266,40 +266,57 @@ extern "C" int sbd_correlation(sbd_ctx* ctx, const double* W, double* onebody, d
   const auto& Bs = ctx->b_host;
 
   // ---- pair records ----
+  // A pair's sums are symmetric in (bra, ket): the products commute and the
+  // spectator mask depends on the orbital ranges only.  So each unordered pair
+  // gets one record (from its lower string); `*_rec[k]` maps table entry k to
+  // its pair's record for either direction.
   // alpha side (M = W, spectator beta): diagonal and singles with T_q, then doubles
   // beta side (M = W^T, spectator alpha): singles with T_q, then diagonal and doubles
   std::vector<RdmRec> ra, rb;
-  ra.reserve(na + ha.s_rec.size() + ha.d_rec.size());
-  rb.reserve(nb + hb.s_rec.size() + hb.d_rec.size());
+  ra.reserve(na + (ha.s_rec.size() + ha.d_rec.size()) / 2);
+  rb.reserve(nb + (hb.s_rec.size() + hb.d_rec.size()) / 2);
   for (uint64_t ia = 0; ia < na; ia++) ra.push_back({(uint32_t)ia, (uint32_t)ia, 0});
-  auto add_singles = [&](std::vector<RdmRec>& out, const HostTab& h, uint64_t n,
-                         const std::vector<uint64_t>& S, int s) {
+  // record of every table entry: entries with ket > bra own one; the reverse
+  // entry (ket < bra) is found in the ket's row (kets ascending)
+  auto pair_records = [&](std::vector<RdmRec>& out, const std::vector<uint32_t>& off,
+                          const std::vector<PairRec>& rec, uint64_t n,
+                          const std::vector<uint64_t>& S, int s, bool dbl) {
+    std::vector<uint64_t> idx(rec.size());
     for (uint64_t r = 0; r < n; r++)
-      for (uint32_t k = h.s_off[r]; k < h.s_off[r + 1]; k++) {
-        const uint32_t ket = h.s_rec[k].ket;
-        const int x = lowbit(S[r] & ~S[ket]), y = lowbit(S[ket] & ~S[r]);
-        out.push_back({(uint32_t)r, ket, spectator_mask(s, std::min(x, y), std::max(x, y))});
-      }
-  };
-  auto add_doubles = [&](std::vector<RdmRec>& out, const HostTab& h, uint64_t n,
-                         const std::vector<uint64_t>& S, int s) {
-    for (uint64_t r = 0; r < n; r++)
-      for (uint32_t k = h.d_off[r]; k < h.d_off[r + 1]; k++) {
-        const uint32_t ket = h.d_rec[k].ket;
+      for (uint32_t k = off[r]; k < off[r + 1]; k++) {
+        const uint32_t ket = rec[k].ket;
+        if (ket < r) continue;
         const uint64_t bo = S[r] & ~S[ket], ko = S[ket] & ~S[r];
-        const int x0 = lowbit(bo), x1 = highbit(bo), y0 = lowbit(ko), y1 = highbit(ko);
-        out.push_back({(uint32_t)r, ket,
-                       spectator_mask(s, std::min(x0, y0), std::max(x0, y0)) ^
-                           spectator_mask(s, std::min(x1, y1), std::max(x1, y1))});
+        uint64_t m;
+        if (dbl) {
+          const int x0 = lowbit(bo), x1 = highbit(bo), y0 = lowbit(ko), y1 = highbit(ko);
+          m = spectator_mask(s, std::min(x0, y0), std::max(x0, y0)) ^
+              spectator_mask(s, std::min(x1, y1), std::max(x1, y1));
+        } else {
+          const int x = lowbit(bo), y = lowbit(ko);
+          m = spectator_mask(s, std::min(x, y), std::max(x, y));
+        }
+        idx[k] = out.size();
+        out.push_back({(uint32_t)r, ket, m});
       }
+    for (uint64_t r = 0; r < n; r++)
+      for (uint32_t k = off[r]; k < off[r + 1]; k++) {
+        const uint32_t ket = rec[k].ket;
+        if (ket > r) continue;
+        const auto b = rec.begin() + off[ket], e = rec.begin() + off[ket + 1];
+        const auto it = std::lower_bound(
+            b, e, (uint32_t)r, [](const PairRec& q, uint32_t v) { return q.ket < v; });
+        idx[k] = idx[it - rec.begin()];
+      }
+    return idx;
   };
-  add_singles(ra, ha, na, As, 0);
+  const std::vector<uint64_t> as_idx = pair_records(ra, ha.s_off, ha.s_rec, na, As, 0, false);
   const uint64_t nq_a = ra.size();
-  add_doubles(ra, ha, na, As, 0);
-  add_singles(rb, hb, nb, Bs, 1);
+  const std::vector<uint64_t> ad_idx = pair_records(ra, ha.d_off, ha.d_rec, na, As, 0, true);
+  const std::vector<uint64_t> bs_idx = pair_records(rb, hb.s_off, hb.s_rec, nb, Bs, 1, false);
   const uint64_t nq_b = rb.size();
   for (uint64_t ib = 0; ib < nb; ib++) rb.push_back({(uint32_t)ib, (uint32_t)ib, 0});
-  add_doubles(rb, hb, nb, Bs, 1);
+  const std::vector<uint64_t> bd_idx = pair_records(rb, hb.d_off, hb.d_rec, nb, Bs, 1, true);
 
   // ---- type 0: alpha single chunks by (x, y), beta singles by (c, d) ----
   std::vector<AbRec> arec;
@@ -465,20 +482,27 @@ extern "C" int sbd_correlation(sbd_ctx* ctx, const double* W, double* onebody, d
     E.add(3 * s, y0, y1, x1, x0, -v);
     E.add(3 * s, y1, y0, x0, x1, -v);
   };
-  for (uint64_t k = na; k < nq_a; k++) {
-    const RdmRec& r = ra[k];
-    one_diff(0, As[r.r1], lowbit(As[r.r1] & ~As[r.r2]), lowbit(As[r.r2] & ~As[r.r1]),
-             &oa[k * stride]);
-  }
-  for (uint64_t k = nq_a; k < ra.size(); k++)
-    two_diff(0, As[ra[k].r1], As[ra[k].r2], oa[nq_a * stride + (k - nq_a)]);
-  for (uint64_t k = 0; k < nq_b; k++) {
-    const RdmRec& r = rb[k];
-    one_diff(1, Bs[r.r1], lowbit(Bs[r.r1] & ~Bs[r.r2]), lowbit(Bs[r.r2] & ~Bs[r.r1]),
-             &ob[k * stride]);
-  }
-  for (uint64_t k = nq_b + nb; k < rb.size(); k++)
-    two_diff(1, Bs[rb[k].r1], Bs[rb[k].r2], ob[nq_b * stride + (k - nq_b)]);
+  // every table entry (both directions of each pair), bra order as the
+  // reference visits them
+  auto singles = [&](int s, const HostTab& h, uint64_t n, const std::vector<uint64_t>& S,
+                     const std::vector<uint64_t>& idx, const std::vector<double>& out) {
+    for (uint64_t r = 0; r < n; r++)
+      for (uint32_t k = h.s_off[r]; k < h.s_off[r + 1]; k++) {
+        const uint32_t ket = h.s_rec[k].ket;
+        one_diff(s, S[r], lowbit(S[r] & ~S[ket]), lowbit(S[ket] & ~S[r]), &out[idx[k] * stride]);
+      }
+  };
+  auto doubles = [&](int s, const HostTab& h, uint64_t n, const std::vector<uint64_t>& S,
+                     const std::vector<uint64_t>& idx, const std::vector<double>& out,
+                     uint64_t nq) {
+    for (uint64_t r = 0; r < n; r++)
+      for (uint32_t k = h.d_off[r]; k < h.d_off[r + 1]; k++)
+        two_diff(s, S[r], S[h.d_rec[k].ket], out[nq * stride + (idx[k] - nq)]);
+  };
+  singles(0, ha, na, As, as_idx, oa);
+  doubles(0, ha, na, As, ad_idx, oa, nq_a);
+  singles(1, hb, nb, Bs, bs_idx, ob);
+  doubles(1, hb, nb, Bs, bd_idx, ob, nq_b);
   // type 0: chunks of equal (x, y) are consecutive; add them in order, then
   // write the reference's entries (both of its branches: spin(I) == spin(A)
   // and spin(I) == spin(B), the latter into block spin(I) + 2 spin(J) as the

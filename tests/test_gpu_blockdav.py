@@ -66,15 +66,16 @@ def test_block_davidson_lowest_roots(torch, L, na, nb, seed, init):
 
 
 def test_block_davidson_mixed_precision(torch):
-    """FP32 block phase to 1e-4, then FP64 to 1e-8: every root within 1e-9 Ha of
-    the FP64 block solve (and of the exact eigenvalues)."""
+    """FP32 block phase to 1e-4, then FP64 to 1e-7: every root within 1e-9 Ha of
+    the FP64 block solve (and of the exact eigenvalues; at residual 1e-7 the
+    energies are converged far below that)."""
     import sbd_amd
     say("mixed precision")
     w, H = dense_case(7, 3, 3, 53)
     ev = np.linalg.eigvalsh(H)
     m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
-    opt64 = sbd_amd.SBD(max_it=400, max_nb=16, eps=1e-8, init=1, seed=3)
-    opt32 = sbd_amd.SBD(max_it=400, max_nb=16, eps=1e-8, init=1, seed=3, precision=1,
+    opt64 = sbd_amd.SBD(max_it=400, max_nb=16, eps=1e-7, init=1, seed=3)
+    opt32 = sbd_amd.SBD(max_it=400, max_nb=16, eps=1e-7, init=1, seed=3, precision=1,
                         fp32_eps=1e-4)
     E64, _, r64 = m.diag_roots(opt64, 4)
     E32, _, r32 = m.diag_roots(opt32, 4)

@@ -64,8 +64,33 @@ def make_pair_fcidump(mol, alpha):
     return ref, m
 
 
+def custom_case(L, a, b, seed, rank):
+    """Edge-case bases: given strings, synthetic integrals of the given rank."""
+    from sbd_amd import synth
+    I0, h, eri = synth.make_integrals(L, seed, rank)
+    return dict(norb=L, I0=I0, h=h, eri=eri, a=np.asarray(a, dtype=np.uint64),
+                b=np.asarray(b, dtype=np.uint64))
+
+
+def _strings(L, n, count, seed):
+    from sbd_amd import synth
+    return synth.random_strings(L, n, count, seed)
+
+
+def _all(L, n):
+    from sbd_amd import synth
+    return synth.all_strings(L, n)
+
+
 CASES = [
     ("fci12", lambda: make_pair(synth_case("fci12", 1.0))),
+    # edge cases: 64 orbitals (the widest string), one alpha string, an empty
+    # beta string (no beta electrons), and a fully connected FCI space
+    ("l64", lambda: make_pair(custom_case(64, _strings(64, 3, 200, 641), _strings(64, 2, 150, 642),
+                                          6464, 16))),
+    ("one_alpha", lambda: make_pair(custom_case(8, [0b00000111], _all(8, 3), 808, 16))),
+    ("empty_beta", lambda: make_pair(custom_case(8, _all(8, 2), [0], 809, 16))),
+    ("fci6_dense", lambda: make_pair(custom_case(6, _all(6, 3), _all(6, 3), 606, 12))),
     ("sci36_small", lambda: make_pair(synth_case("sci36", 0.04))),
     ("sci48open_small", lambda: make_pair(synth_case("sci48open", 0.02))),
     ("sci58_small", lambda: make_pair(synth_case("sci58", 0.01))),
