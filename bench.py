@@ -20,8 +20,9 @@ Printed JSON line:
   serving      Poisson streams of independent solves (norb 16-40, n 4-10,
                200-8000 strings per spin) through the continuous-batching
                server vs the same stream solved one at a time: p50/p99
-               latency and solves/s, at overload (64 requests at 40/s) and at
-               a steady sub-saturation load (32 requests at 0.5/s)
+               latency and solves/s, at overload (64 requests at 40/s; the
+               server admitting in arrival order and smallest-first) and at a
+               steady sub-saturation load (32 requests at 0.5/s)
 """
 import argparse
 import json
@@ -93,12 +94,12 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
 
 
-def serve_stream(reqs, opt, args, label):
-    """Serve one request stream with continuous batching, then the same stream
-    one request at a time with the plain solver (queueing included)."""
+def serve_batched(reqs, opt, args, label, policy):
+    """Serve one request stream with continuous batching under one admission
+    policy; returns (results, stats)."""
     import threading
     import sbd_amd
-    sv = sbd_amd.Server(args.serve_mem_gb << 30, max_active=args.serve_slots)
+    sv = sbd_amd.Server(args.serve_mem_gb << 30, max_active=args.serve_slots, policy=policy)
     ids = [None] * len(reqs)
     base = sv.now()
 
@@ -117,13 +118,25 @@ def serve_stream(reqs, opt, args, label):
     for q, i in enumerate(ids):
         res.append(sv.wait(i))
         if (q + 1) % 16 == 0:
-            log(f"serving {label}: batched, {q + 1} solves done")
+            log(f"serving {label}: batched ({policy}), {q + 1} solves done")
     st = sv.stats()
     sv.close()
     lat = [r["t_done"] - r["t_submit"] for r in res]
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
-    log(f"serving {label}: batched p50 {pct(lat, 50):.3f}s p99 {pct(lat, 99):.3f}s "
+    log(f"serving {label}: batched ({policy}) p50 {pct(lat, 50):.3f}s p99 {pct(lat, 99):.3f}s "
         f"{len(res) / span:.2f} solves/s peak active {st['peak_active']}")
+    return res, {"solves_per_s": round(len(res) / span, 3), "p50_s": round(pct(lat, 50), 4),
+                 "p99_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3),
+                 "peak_active": st["peak_active"], "rounds": st["rounds"],
+                 "converged": sum(r["converged"] for r in res)}
+
+
+def serve_stream(reqs, opt, args, label, policies):
+    """Serve one request stream with continuous batching (one run per admission
+    policy), then the same stream one request at a time with the plain solver
+    (arrival order, queueing included)."""
+    import sbd_amd
+    batched = {p: serve_batched(reqs, opt, args, label, p) for p in policies}
     seq_lat, seq_E = [], []
     t_start = time.time()
     for r in reqs:
@@ -140,21 +153,16 @@ def serve_stream(reqs, opt, args, label):
     seq_span = time.time() - t_start - reqs[0]["t"]
     log(f"serving {label}: sequential p50 {pct(seq_lat, 50):.3f}s p99 {pct(seq_lat, 99):.3f}s "
         f"{len(reqs) / seq_span:.2f} solves/s")
-    same = sum(1 for a, b in zip(res, seq_E) if a["energy"] == b)
-    return {
-        "requests": len(reqs),
-        "batched": {"solves_per_s": round(len(res) / span, 3), "p50_s": round(pct(lat, 50), 4),
-                    "p99_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3),
-                    "peak_active": st["peak_active"], "rounds": st["rounds"],
-                    "converged": sum(r["converged"] for r in res)},
-        "sequential_plain": {"solves_per_s": round(len(reqs) / seq_span, 3),
-                             "p50_s": round(pct(seq_lat, 50), 4),
-                             "p99_s": round(pct(seq_lat, 99), 4),
-                             "makespan_s": round(seq_span, 3)},
-        "throughput_ratio": round((len(res) / span) / (len(reqs) / seq_span), 3),
-        "p50_ratio_sequential_over_batched": round(pct(seq_lat, 50) / max(pct(lat, 50), 1e-9), 3),
-        "energies_bit_identical": f"{same}/{len(reqs)}",
-    }
+    seq = {"solves_per_s": round(len(reqs) / seq_span, 3), "p50_s": round(pct(seq_lat, 50), 4),
+           "p99_s": round(pct(seq_lat, 99), 4), "makespan_s": round(seq_span, 3)}
+    out = {"requests": len(reqs), "sequential_plain": seq}
+    for p, (res, b) in batched.items():
+        same = sum(1 for a, e in zip(res, seq_E) if a["energy"] == e)
+        b["throughput_ratio"] = round(b["solves_per_s"] / seq["solves_per_s"], 3)
+        b["p50_ratio_sequential_over_batched"] = round(seq["p50_s"] / max(b["p50_s"], 1e-9), 3)
+        b["energies_bit_identical"] = f"{same}/{len(reqs)}"
+        out[f"batched_{p}"] = b
+    return out
 
 
 def serving(args):
@@ -164,22 +172,24 @@ def serving(args):
     "overload" (arrivals far faster than the GPU completes solves: throughput)
     and "steady" (about half the sequential service rate: latency, where a
     small solve that arrives behind a large one waits for it when solves run
-    one at a time)."""
+    one at a time).  Batched runs use the server's admission policies "fifo"
+    (arrival order) and "sjf" (smallest footprint first, with aging); the
+    sequential baseline takes requests in arrival order."""
     import sbd_amd
     from sbd_amd import synth
     opt = sbd_amd.SBD(max_it=args.serve_max_it, max_nb=10, eps=args.serve_eps, init=0)
     out = {"options": {"max_it": args.serve_max_it, "max_nb": 10, "eps": args.serve_eps,
                        "init": 0}}
-    for label, count, rate, seed in (("overload", args.serve_count, args.serve_rate, 4004),
-                                     ("steady", args.serve_steady_count, args.serve_steady_rate,
-                                      4005)):
+    for label, count, rate, seed, policies in (
+            ("overload", args.serve_count, args.serve_rate, 4004, ("fifo", "sjf")),
+            ("steady", args.serve_steady_count, args.serve_steady_rate, 4005, ("sjf",))):
         if count <= 0:
             continue
         t0 = time.time()
         reqs = synth.serving_requests(count, seed, rate=rate)
         log(f"serving {label}: {len(reqs)} requests at {rate}/s generated in "
             f"{time.time() - t0:.1f}s, N total {sum(len(r['a']) * len(r['b']) for r in reqs):.3e}")
-        out[label] = serve_stream(reqs, opt, args, label)
+        out[label] = serve_stream(reqs, opt, args, label, policies)
         out[label]["arrival_rate_per_s"] = rate
     return out
 

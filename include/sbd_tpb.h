@@ -348,6 +348,11 @@ int sbd_server_submit(sbd_server* sv, const sbd_request* rq, uint64_t* id);
 int sbd_server_wait(sbd_server* sv, uint64_t id, sbd_result* out);
 int sbd_server_poll(sbd_server* sv, uint64_t id, sbd_result* out, int* done);
 double sbd_server_now(const sbd_server* sv);
+/* Admission order (no counterpart in the reference: a serving addition).
+ * policy 0 (default): arrival order.  policy 1: the queued request with the
+ * smallest device footprint first, except that a request that has waited
+ * age_limit_s seconds or more goes first.  Results do not depend on it. */
+int sbd_server_set_policy(sbd_server* sv, int policy, double age_limit_s);
 /* out[0] = engine rounds so far, out[1] = most solves active in one round. */
 int sbd_server_stats(const sbd_server* sv, uint64_t* out);
 /* Finishes every queued solve, then frees the server. */

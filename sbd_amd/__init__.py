@@ -39,6 +39,7 @@ EXPORTS = [
     "sbd_encode_strings_bin",
     "sbd_diag", "sbd_diag_roots", "sbd_diag_footprint", "sbd_server_create", "sbd_server_submit",
     "sbd_server_wait", "sbd_server_poll", "sbd_server_now", "sbd_server_stats",
+    "sbd_server_set_policy",
     "sbd_server_destroy",
 ]
 
@@ -141,6 +142,7 @@ def lib():
     L.sbd_server_now.restype = C.c_double
     L.sbd_server_now.argtypes = [vp]
     L.sbd_server_stats.argtypes = [vp, up]
+    L.sbd_server_set_policy.argtypes = [vp, C.c_int, C.c_double]
     L.sbd_server_destroy.argtypes = [vp]
     _LIB = L
     return L
@@ -613,14 +615,21 @@ class Server:
     once; ``wait`` blocks until that solve is done and returns its result
     (energy, residual, converged, steps, status, and the submit / admit / done
     times on the server clock).  A solve's result is bit-identical alone, in
-    any mix, and to ``MultTPB.diag`` with the same options."""
+    any mix, and to ``MultTPB.diag`` with the same options.  ``policy`` is the
+    admission order: "fifo" (arrival order) or "sjf" (smallest footprint
+    first; a request waiting ``age_limit`` seconds goes first)."""
 
-    def __init__(self, mem_budget_bytes: int, max_active: int = 32):
+    def __init__(self, mem_budget_bytes: int, max_active: int = 32, policy: str = "fifo",
+                 age_limit: float = 30.0):
+        if policy not in ("fifo", "sjf"):
+            raise SbdError("policy must be 'fifo' or 'sjf'")
         st = C.c_int(0)
         h = lib().sbd_server_create(int(mem_budget_bytes), int(max_active), C.byref(st))
         if not h:
             _check(st.value if st.value else 1)
         self._h = C.c_void_p(h)
+        _check(lib().sbd_server_set_policy(self._h, 0 if policy == "fifo" else 1,
+                                           float(age_limit)))
 
     def submit(self, norb, a, b, *, I0=0.0, h=None, eri=None, sbd: "SBD | None" = None) -> int:
         sbd = sbd or SBD()

@@ -161,8 +161,23 @@ struct sbd_server {
   double t0 = 0.0;
   uint64_t rounds = 0, peak_active = 0;
   std::string last_error;
+  // admission order: 0 = arrival order; 1 = smallest footprint first, except
+  // that a request waiting age_limit seconds or more goes first (no request
+  // waits behind smaller ones for ever)
+  int policy = 0;
+  double age_limit = 30.0;
 
   double now() const { return clock_s() - t0; }
+
+  // index of the pending request to admit next (mu held, pending not empty;
+  // pending is in arrival order)
+  size_t pick_pending() const {
+    if (policy == 0 || now() - pending.front().t_submit >= age_limit) return 0;
+    size_t best = 0;
+    for (size_t i = 1; i < pending.size(); i++)
+      if (pending[i].bytes < pending[best].bytes) best = i;
+    return best;
+  }
 
   void finish(Solve* s, double energy) {
     sbd_result r{};
@@ -239,14 +254,15 @@ struct sbd_server {
         cv_admit.wait(g, [&] {
           if (stopping && pending.empty()) return true;
           if (pending.empty()) return false;
-          const uint64_t need = pending.front().bytes;
+          const uint64_t need = pending[pick_pending()].bytes;
           const bool slot = running + preparing + queued < max_active;
           const bool mem = used + need <= budget || (running + preparing + queued == 0);
           return slot && mem;
         });
         if (pending.empty()) return;  // stopping
-        q = std::move(pending.front());
-        pending.pop_front();
+        const size_t k = pick_pending();
+        q = std::move(pending[k]);
+        pending.erase(pending.begin() + (std::ptrdiff_t)k);
         used += q.bytes;
         preparing++;
         state[q.id] = 1;
@@ -537,6 +553,15 @@ int sbd_server_poll(sbd_server* sv, uint64_t id, sbd_result* out, int* done) {
 }
 
 double sbd_server_now(const sbd_server* sv) { return sv ? sv->now() : 0.0; }
+
+int sbd_server_set_policy(sbd_server* sv, int policy, double age_limit_s) {
+  if (!sv || (policy != 0 && policy != 1) || !(age_limit_s > 0.0)) return SBD_ERR_ARG;
+  std::lock_guard<std::mutex> g(sv->mu);
+  sv->policy = policy;
+  sv->age_limit = age_limit_s;
+  sv->cv_admit.notify_all();
+  return SBD_OK;
+}
 
 int sbd_server_stats(const sbd_server* sv, uint64_t* out) {
   if (!sv || !out) return SBD_ERR_ARG;

@@ -108,3 +108,32 @@ def test_server_rejects_bad_requests(torch, requests_):
         sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"],
                   sbd=sbd_amd.SBD(precision=1))
     sv.close()
+
+
+def test_server_sjf_admission(torch):
+    """policy "sjf": with room for one solve at a time, every request admitted
+    after the first is the smallest one still queued (same norb and electron
+    count, string counts 40..200, so the footprint grows with the count), and
+    the energies equal those of arrival-order admission."""
+    import sbd_amd
+    from sbd_amd import synth
+    I0, h, eri = synth.make_integrals(12, 1212, 24)
+    counts = [120, 40, 200, 80, 160]  # submission order
+    rs = [dict(norb=12, I0=I0, h=h, eri=eri, a=synth.random_strings(12, 4, c, 30 + c),
+               b=synth.random_strings(12, 4, c, 60 + c)) for c in counts]
+    results = {}
+    for policy in ("fifo", "sjf"):
+        sv = sbd_amd.Server(1, max_active=8, policy=policy)
+        ids = [sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"],
+                         sbd=opts()) for r in rs]
+        results[policy] = [sv.wait(i) for i in ids]
+        sv.close()
+    for a, b in zip(results["fifo"], results["sjf"]):
+        assert a["energy"] == b["energy"] and a["steps"] == b["steps"]
+    res = results["sjf"]
+    order = sorted(range(len(rs)), key=lambda q: res[q]["t_admit"])
+    say(f"sjf admission order by string count: {[counts[q] for q in order]}")
+    for i in range(1, len(order)):
+        assert counts[order[i]] == min(counts[q] for q in order[i:]), order
+    with pytest.raises(sbd_amd.SbdError):
+        sbd_amd.Server(1, policy="lifo")
