@@ -446,41 +446,38 @@ extern "C" int sbd_correlation(sbd_ctx* ctx, const double* W, double* onebody, d
       E.add(2, p, q, p, q, z);
       E.add(1, q, p, q, p, z);
     }
-  // OneDiffCorrelation per single record of spin s: bra string S, bra-only x,
-  // ket-only y; same-spin spectators from S (x included), other-spin
-  // spectators weighted by T_q
+  // OneDiffCorrelation per single excitation of spin s (bra string S, bra-only
+  // x, ket-only y): the one-body entry, then the two-body entries of each
+  // same-spin spectator oj in S (x included) with weight T0 and of each
+  // other-spin spectator q with weight T_q.  The two-body weights are first
+  // summed per (x, y, oj) and (x, y, q) (L^3 each), then written out.
+  std::vector<double> A3(2 * L3, 0.0), B3(2 * L3, 0.0);
   auto one_diff = [&](int s, uint64_t S, int x, int y, const double* o) {
     const double sg = (__builtin_popcountll(S & open_range(std::min(x, y), std::max(x, y))) & 1)
                           ? -1.0 : 1.0;
     const double v0 = sg * o[0];
     onebody[s * L2 + x + (size_t)L * y] += v0;
-    for (uint64_t u = S; u; u &= u - 1) {
-      const int oj = lowbit(u);
-      E.add(3 * s, y, oj, x, oj, v0);
-      E.add(3 * s, oj, y, oj, x, v0);
-      E.add(3 * s, y, oj, oj, x, -v0);
-      E.add(3 * s, oj, y, x, oj, -v0);
-    }
-    const int so = 1 - s;
-    for (int q = 0; q < L; q++) {
-      const double vq = sg * o[1 + q];
-      E.add(s + 2 * so, y, q, x, q, vq);
-      E.add(so + 2 * s, q, y, q, x, vq);
-    }
+    double* a3 = &A3[s * L3 + ((size_t)x * L + y) * L];
+    for (uint64_t u = S; u; u &= u - 1) a3[lowbit(u)] += v0;
+    double* b3 = &B3[s * L3 + ((size_t)x * L + y) * L];
+    for (int q = 0; q < L; q++) b3[q] += sg * o[1 + q];
   };
-  // TwoDiffCorrelation per same-spin double record: bra-only x0 < x1,
-  // ket-only y0 < y1
+  // TwoDiffCorrelation per same-spin double excitation (bra-only x0 < x1,
+  // ket-only y0 < y1): the signed sum is kept per orbital quadruple, D[s]
+  // indexed by the pairs (x0, x1) and (y0, y1)
+  std::vector<uint32_t> pidx(L2, 0);
+  uint32_t npair = 0;
+  for (int a = 0; a < L; a++)
+    for (int b = a + 1; b < L; b++) pidx[(size_t)a * L + b] = npair++;
+  std::vector<double> D(2 * (size_t)npair * npair, 0.0);
   auto two_diff = [&](int s, uint64_t S, uint64_t K, double t0) {
     const uint64_t bo = S & ~K, ko = K & ~S;
     const int x0 = lowbit(bo), x1 = highbit(bo), y0 = lowbit(ko), y1 = highbit(ko);
     int par = __builtin_popcountll(S & open_range(std::min(x0, y0), std::max(x0, y0))) +
               __builtin_popcountll(S & open_range(std::min(x1, y1), std::max(x1, y1)));
     if (y0 > x1 || y1 < x0) par++;
-    const double v = (par & 1) ? -t0 : t0;
-    E.add(3 * s, y0, y1, x0, x1, v);
-    E.add(3 * s, y1, y0, x1, x0, v);
-    E.add(3 * s, y0, y1, x1, x0, -v);
-    E.add(3 * s, y1, y0, x0, x1, -v);
+    D[(size_t)s * npair * npair + (size_t)pidx[(size_t)x0 * L + x1] * npair +
+      pidx[(size_t)y0 * L + y1]] += (par & 1) ? -t0 : t0;
   };
   // every table entry (both directions of each pair), bra order as the
   // reference visits them
@@ -503,6 +500,45 @@ extern "C" int sbd_correlation(sbd_ctx* ctx, const double* W, double* onebody, d
   doubles(0, ha, na, As, ad_idx, oa, nq_a);
   singles(1, hb, nb, Bs, bs_idx, ob);
   doubles(1, hb, nb, Bs, bd_idx, ob, nq_b);
+  for (int s = 0; s < 2; s++) {
+    const int so = 1 - s;
+    for (int x = 0; x < L; x++)
+      for (int y = 0; y < L; y++) {
+        const double* a3 = &A3[s * L3 + ((size_t)x * L + y) * L];
+        const double* b3 = &B3[s * L3 + ((size_t)x * L + y) * L];
+        for (int j = 0; j < L; j++) {
+          if (a3[j] != 0.0) {
+            E.add(3 * s, y, j, x, j, a3[j]);
+            E.add(3 * s, j, y, j, x, a3[j]);
+            E.add(3 * s, y, j, j, x, -a3[j]);
+            E.add(3 * s, j, y, x, j, -a3[j]);
+          }
+          if (b3[j] != 0.0) {
+            E.add(s + 2 * so, y, j, x, j, b3[j]);
+            E.add(so + 2 * s, j, y, j, x, b3[j]);
+          }
+        }
+      }
+    // same-spin block: entry (p0, p1, p2, p3) takes the quadruple with
+    // (x0, x1) = the pair {p2, p3} and (y0, y1) = {p0, p1}: +D for
+    // (y0, y1, x0, x1) and (y1, y0, x1, x0), -D for (y0, y1, x1, x0) and
+    // (y1, y0, x0, x1); written in index order
+    const double* Ds = &D[(size_t)s * npair * npair];
+    double* blk = twobody + (size_t)(3 * s) * L4;
+    for (int p3 = 0; p3 < L; p3++)
+      for (int p2 = 0; p2 < L; p2++) {
+        if (p2 == p3) continue;
+        const size_t px = pidx[(size_t)std::min(p2, p3) * L + std::max(p2, p3)];
+        const double* row = Ds + px * npair;
+        double* dst = blk + (size_t)L2 * p2 + L3 * p3;
+        for (int p1 = 0; p1 < L; p1++)
+          for (int p0 = 0; p0 < L; p0++) {
+            if (p0 == p1) continue;
+            const double v = row[pidx[(size_t)std::min(p0, p1) * L + std::max(p0, p1)]];
+            dst[p0 + (size_t)L * p1] += ((p0 < p1) == (p2 < p3)) ? v : -v;
+          }
+      }
+  }
   // type 0: chunks of equal (x, y) are consecutive; add them in order, then
   // write the reference's entries (both of its branches: spin(I) == spin(A)
   // and spin(I) == spin(B), the latter into block spin(I) + 2 spin(J) as the
