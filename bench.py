@@ -39,6 +39,7 @@ WORKLOAD_DESC = {
     "sci36": "norb=36 na=nb=8, 20000x20000 strings (4.0e8 dets), ERI rank 72",
     "fci12": "norb=12 na=nb=4, FCI 495x495 strings (245,025 dets), ERI rank 24",
     "sci58": "norb=58 na=nb=10, 60000x60000 strings (3.6e9 dets), ERI rank 116",
+    "sci48open": "norb=48 na=12 nb=10, 30000x25000 strings (7.5e8 dets), ERI rank 96",
 }
 
 
