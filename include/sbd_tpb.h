@@ -143,6 +143,13 @@ sbd_ctx* sbd_ctx_create(int norb, double I0, const double* h1, const double* eri
                         size_t nb, int* status);
 void sbd_ctx_destroy(sbd_ctx* ctx);
 
+/* Sigma kernel choice (no counterpart in the reference: a tuning knob of this
+ * library).  0 (default): by sparsity -- bases with fewer than 2 off-diagonal
+ * pairs per alpha string plus per beta string use the per-element kernel,
+ * others the row kernel; 1: row kernel; 2: per-element kernel.  Both give the
+ * same bits. */
+int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel);
+
 /* Dimensions: out[0] = na, out[1] = nb, out[2] = N = na*nb, out[3] = norb. */
 int sbd_ctx_dims(const sbd_ctx* ctx, uint64_t* out);
 

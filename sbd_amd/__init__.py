@@ -30,7 +30,8 @@ _LIB = None
 EXPORTS = [
     "sbd_status_string", "sbd_last_error", "sbd_options_default", "sbd_pack_strings",
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
-    "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_make_helpers", "sbd_table_counts", "sbd_table_get",
+    "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_set_sigma_kernel", "sbd_make_helpers",
+    "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
     "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_correlation", "sbd_carryover",
     "sbd_diag_from",
@@ -105,6 +106,7 @@ def lib():
     L.sbd_ctx_create.argtypes = [C.c_int, C.c_double, dp, dp, up, C.c_size_t, up, C.c_size_t, ip]
     L.sbd_ctx_destroy.argtypes = [vp]
     L.sbd_ctx_dims.argtypes = [vp, up]
+    L.sbd_set_sigma_kernel.argtypes = [vp, C.c_int]
     L.sbd_make_helpers.argtypes = [vp]
     L.sbd_table_counts.argtypes = [vp, up]
     L.sbd_table_get.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -343,6 +345,11 @@ class MultTPB:
 
     def make_helpers(self):
         _check(lib().sbd_make_helpers(self._ctx), self._ctx)
+
+    def set_sigma_kernel(self, kernel: str = "auto"):
+        """Sigma kernel: "auto" (by sparsity), "rows" or "elements"; same results."""
+        k = {"auto": 0, "rows": 1, "elements": 2}[kernel]
+        _check(lib().sbd_set_sigma_kernel(self._ctx, k), self._ctx)
 
     def table_counts(self):
         out = np.zeros(4, dtype=np.uint64)

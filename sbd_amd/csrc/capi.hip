@@ -557,6 +557,12 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
   delete ctx;
 }
 
+int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel) {
+  if (!ctx || kernel < 0 || kernel > 2) return SBD_ERR_ARG;
+  ctx->sigma_kernel = kernel;
+  return SBD_OK;
+}
+
 int sbd_ctx_dims(const sbd_ctx* ctx, uint64_t* out) {
   if (!ctx || !out) return SBD_ERR_ARG;
   out[0] = ctx->na;

@@ -226,6 +226,7 @@ struct sbd_ctx {
   size_t partial_cap = 0;
   int lds_max = 65536;
   int cu_count = 256;
+  int sigma_kernel = 0;  // 0 = by sparsity, 1 = row kernel, 2 = per-element kernel
   std::string last_error;
 };
 

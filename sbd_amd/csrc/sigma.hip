@@ -674,6 +674,119 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* _
   sigma_rows_body<VT, true, true, kRows, true>(J.p, blockIdx.x - J.blk0, J.ngrp, J.dot_part);
 }
 
+// Sparse bases (a few off-diagonal terms per determinant): one element per
+// thread, no row staging, every wavefront streaming.  Workgroup tiles are
+// 512 consecutive columns of one row; thread t owns columns base + t and
+// base + t + 256.  Each element is summed in the order of k_sigma_rows
+// (diagonal, alpha singles, alpha doubles, beta singles, beta doubles from the
+// CSR in the order of the ELL slots, alpha-single x beta-single terms) with the
+// same operations, so the results are the same bits.
+constexpr int kElemBlock = 256;
+constexpr double kSparseTerms = 2.0;  // off-diagonal pairs per alpha string + per beta string
+constexpr int kElemCols = 2;
+constexpr int kElemTile = kElemBlock * kElemCols;
+
+template <typename VT, bool kCache>
+__global__ __launch_bounds__(kElemBlock) void k_sigma_elems(const SigmaArgs<VT> p,
+                                                            const PairRec* __restrict__ bd_rec,
+                                                            uint64_t ntiles,
+                                                            double* __restrict__ dot_part) {
+  using AT = VT;
+  const uint32_t nb = (uint32_t)p.nb;
+  const int L = p.L;
+  const uint64_t tpr = (nb + kElemTile - 1) / kElemTile;  // tiles per row
+  double tw = 0.0;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t ia = tile / tpr;
+    const uint32_t base = (uint32_t)(tile - ia * tpr) * kElemTile;
+    const uint64_t A = p.a[ia];
+    const VT* __restrict__ Trow = p.T + ia * nb;
+    uint32_t col[kElemCols];
+    bool ok[kElemCols];
+    ColRec cb[kElemCols];
+    AT acc[kElemCols];
+#pragma unroll
+    for (int j = 0; j < kElemCols; j++) {
+      const uint32_t c = base + threadIdx.x + j * kElemBlock;
+      ok[j] = c < nb;
+      col[j] = ok[j] ? c : nb - 1;
+      cb[j] = p.b_rec[col[j]];
+      const VT hv = p.hii[ia * nb + col[j]];
+      const VT ts = Trow[col[j]];
+      acc[j] = p.accumulate ? (AT)p.W[ia * nb + col[j]] : (AT)0;
+      acc[j] += (AT)hv * (AT)ts;
+    }
+    // alpha singles (uniform over the workgroup)
+    const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
+    for (uint32_t q = as0; q < as1; q++) {
+      const PairRec rc = p.as_rec[q];
+      const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
+#pragma unroll
+      for (int j = 0; j < kElemCols; j++) {
+        const VT tk = Tk[col[j]];
+        const double e = kCache ? p.Ea[(uint64_t)q * nb + col[j]]
+                                : alpha_single_elem(rc, A, cb[j].B, L, p.G);
+        acc[j] += (AT)e * (AT)tk;
+      }
+    }
+    // alpha doubles
+    for (uint32_t q = p.ad_off[ia]; q < p.ad_off[ia + 1]; q++) {
+      const PairRec rc = p.ad_rec[q];
+      const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
+      const uint64_t mm = beta_mask_double(rc.code);
+#pragma unroll
+      for (int j = 0; j < kElemCols; j++) {
+        const VT tv = Tk[col[j]];
+        const double e = flip_sign(rc.val, (uint32_t)__popcll(cb[j].B & mm));
+        acc[j] += (AT)e * (AT)tv;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kElemCols; j++) {
+      // beta singles
+      for (uint32_t q = cb[j].s0; q < cb[j].s1; q++) {
+        const PairRec rc = p.bs_rec[q];
+        const double e = kCache ? p.Eb[ia * (uint64_t)p.Sb + q]
+                                : beta_single_elem(rc, A, cb[j].B, L, p.G);
+        acc[j] += (AT)e * (AT)Trow[rc.ket];
+      }
+      // beta doubles: the value as the ELL slot of this precision holds it
+      for (uint32_t q = p.bd_off[col[j]]; q < p.bd_off[col[j] + 1]; q++) {
+        const PairRec rc = bd_rec[q];
+        const uint64_t mask = bits_range(code_f(rc.code, 0) + 1, code_f(rc.code, 1) + 1) ^
+                              bits_range(code_f(rc.code, 2) + 1, code_f(rc.code, 3) + 1);
+        const VT e = flip_sign((VT)rc.val, (uint32_t)__popcll(A & mask));
+        acc[j] += e * Trow[rc.ket];
+      }
+      // alpha single x beta single
+      if (as1 > as0 && cb[j].s1 > cb[j].s0) {
+        const uint64_t D0 = spread32(A) | (spread32(cb[j].B) << 1);
+        const uint64_t D1 = spread32(A >> 32) | (spread32(cb[j].B >> 32) << 1);
+        for (uint32_t q = as0; q < as1; q++) {
+          const PairRec ra = p.as_rec[q];
+          const int xa = code_x0(ra.code), ya = code_y0(ra.code);
+          const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
+          for (uint32_t k = cb[j].s0; k < cb[j].s1; k++) {
+            const PairRec rb = p.bs_rec[k];
+            const double e = ab_elem(D0, D1, xa, ya, code_x0(rb.code), code_y0(rb.code), L, p.Vab);
+            acc[j] = acc[j] + (AT)e * (AT)Tk[rb.ket];
+          }
+        }
+      }
+      if (ok[j]) {
+        p.W[ia * nb + col[j]] = (VT)acc[j];
+        if (dot_part) tw += (double)Trow[col[j]] * (double)(VT)acc[j];
+      }
+    }
+  }
+  if (dot_part) {
+    // per-wavefront partial, fixed order of lanes
+    for (int o = 32; o > 0; o >>= 1) tw += __shfl_xor(tw, o, 64);
+    if ((threadIdx.x & 63) == 0)
+      dot_part[(uint64_t)blockIdx.x * (kElemBlock / 64) + (threadIdx.x >> 6)] = tw;
+  }
+}
+
 // Sum of n partials in a fixed order: thread t adds partials t, t + 1024, ...
 // (compensated), then the 1024 thread sums are combined in a fixed tree.
 constexpr int kSumBlock = 1024;
@@ -855,6 +968,56 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
     if (x < w1 && w0 < x + sizeof(VT) * ctx->N) {
       ctx->last_error = "sigma: Wb overlaps Wk or hii";
       return SBD_ERR_ARG;
+    }
+  }
+  // Sparse bases (fewer than kSparseTerms off-diagonal pairs per string of
+  // each spin): the per-element kernel, same results
+  {
+    const SpinTables& A = ctx->ta;
+    const SpinTables& B = ctx->tb;
+    const double terms = (double)(A.n_s + A.n_d) / (double)ctx->na +
+                         (double)(B.n_s + B.n_d) / (double)ctx->nb;
+    const bool elems =
+        ctx->sigma_kernel == 2 || (ctx->sigma_kernel == 0 && terms < kSparseTerms);
+    if (elems) {
+      const uint64_t ntiles = ctx->na * ((ctx->nb + kElemTile - 1) / kElemTile);
+      const unsigned eg = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->cu_count * 32);
+      double* epart = nullptr;
+      if (tw_dot) {
+        const size_t need = (size_t)eg * (kElemBlock / 64) + 1;
+        if (ctx->sdot_cap < need) {
+          if (ctx->d_sdot) (void)hipFree(ctx->d_sdot);
+          ctx->d_sdot = nullptr;
+          ctx->sdot_cap = 0;
+          SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_sdot, sizeof(double) * need));
+          ctx->sdot_cap = need;
+        }
+        epart = ctx->d_sdot;
+      }
+      const SigmaArgs<VT> args{ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W,
+                               accumulate, A.s_off, A.s_rec, A.d_off, A.d_rec, B.s_off,
+                               B.s_rec, B.d_off, B.ell64, B.ell32, B.ell_ket, B.ell_off,
+                               B.ell_pos, B.col_rec, (const double*)ctx->d_Ea,
+                               (const double*)ctx->d_Eb, (uint32_t)B.n_s,
+                               (const double*)ctx->d_G, (const double*)ctx->d_eri,
+                               (const double*)ctx->d_Vab};
+      const bool ecache = (A.n_s == 0 || ctx->d_Ea) && (B.n_s == 0 || ctx->d_Eb);
+      if (ecache)
+        hipLaunchKernelGGL((k_sigma_elems<VT, true>), dim3(eg), dim3(kElemBlock), 0, st, args,
+                           B.d_rec, ntiles, epart);
+      else
+        hipLaunchKernelGGL((k_sigma_elems<VT, false>), dim3(eg), dim3(kElemBlock), 0, st, args,
+                           B.d_rec, ntiles, epart);
+      SBD_HIP_TRY(ctx, hipGetLastError());
+      if (tw_dot) {
+        const uint64_t np = (uint64_t)eg * (kElemBlock / 64);
+        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kSumBlock), 0, st, epart, np, epart + np);
+        SBD_HIP_TRY(ctx, hipGetLastError());
+        SBD_HIP_TRY(ctx,
+                    hipMemcpyAsync(tw_dot, epart + np, sizeof(double), hipMemcpyDeviceToHost, st));
+        SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
+      }
+      return SBD_OK;
     }
   }
   const size_t row_bytes = sizeof(VT) * ctx->nb;

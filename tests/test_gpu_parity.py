@@ -137,6 +137,17 @@ def test_tables_hdiag_sigma_bitexact(torch, name, factory):
     W32 = m.sigma(h32, Wk.float())
     err = np.abs(W32.cpu().numpy().astype(np.float64) - s_ref).max() / np.abs(s_ref).max()
     assert err < 2e-5, err
+    say(f"case {name}: both sigma kernels")
+    # the row kernel and the per-element kernel give the same bits, FP64
+    # (overwrite and accumulate) and FP32
+    for kernel in ("rows", "elements"):
+        m.set_sigma_kernel(kernel)
+        assert np.array_equal(m.sigma(hii, Wk).cpu().numpy(), s_ref), kernel
+        Wacc2 = torch.tensor(W0, device="cuda")
+        m.run(hii, Wk, Wacc2)
+        assert np.array_equal(Wacc2.cpu().numpy(), s_acc), kernel
+        assert torch.equal(m.sigma(h32, Wk.float()), W32), kernel
+    m.set_sigma_kernel("auto")
     m.close()
 
 
