@@ -676,14 +676,14 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* _
 
 // Sparse bases (a few off-diagonal terms per determinant): one element per
 // thread, no row staging, every wavefront streaming.  Workgroup tiles are
-// 512 consecutive columns of one row; thread t owns columns base + t and
-// base + t + 256.  Each element is summed in the order of k_sigma_rows
+// 1024 consecutive columns of one row; thread t owns columns base + t + 256 j,
+// j = 0..3.  Each element is summed in the order of k_sigma_rows
 // (diagonal, alpha singles, alpha doubles, beta singles, beta doubles from the
 // CSR in the order of the ELL slots, alpha-single x beta-single terms) with the
 // same operations, so the results are the same bits.
 constexpr int kElemBlock = 256;
 constexpr double kSparseTerms = 2.0;  // off-diagonal pairs per alpha string + per beta string
-constexpr int kElemCols = 2;
+constexpr int kElemCols = 4;
 constexpr int kElemTile = kElemBlock * kElemCols;
 
 template <typename VT, bool kCache>
