@@ -245,7 +245,7 @@ constexpr int kABatch = 4;       // beta singles per alpha single fetched togeth
 // beta double slots of one position fetched together: fewer when the partial
 // sums of several rows share the registers
 template <typename VT, int kRows>
-constexpr int kBBatch = sizeof(VT) * kRows >= 16 ? 2 : sizeof(VT) * kRows >= 8 ? 4 : 8;
+constexpr int kBBatch = sizeof(VT) * kRows >= 16 ? 2 : sizeof(VT) * kRows >= 8 ? 5 : 8;
 
 template <typename VT>
 __device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ Trow, uint32_t nb) {
