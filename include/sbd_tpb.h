@@ -147,7 +147,7 @@ void sbd_ctx_destroy(sbd_ctx* ctx);
  * library).  0 (default): by sparsity -- bases with fewer than 2 off-diagonal
  * pairs per alpha string plus per beta string use the per-element kernel,
  * others the row kernel; 1: row kernel; 2: per-element kernel.  Both give the
- * same bits. */
+ * same bits, and the serving engine's batches follow the same rule. */
 int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel);
 
 /* Dimensions: out[0] = na, out[1] = nb, out[2] = N = na*nb, out[3] = norb. */

@@ -247,18 +247,28 @@ constexpr int kABatch = 4;       // beta singles per alpha single fetched togeth
 template <typename VT, int kRows>
 constexpr int kBBatch = sizeof(VT) * kRows >= 16 ? 2 : sizeof(VT) * kRows >= 8 ? 5 : 8;
 
+// Bra row staged in LDS.  A thread issues up to kStageLoads loads before its
+// stores, so a 20,000-element row takes two memory round trips, not five.
+constexpr int kStageLoads = 16;
 template <typename VT>
 __device__ inline void stage_row(VT* __restrict__ trow, const VT* __restrict__ Trow, uint32_t nb) {
   uint32_t j = threadIdx.x;
-  for (; j + 3 * kSigBlock < nb; j += 4 * kSigBlock) {
-    const VT v0 = Trow[j], v1 = Trow[j + kSigBlock], v2 = Trow[j + 2 * kSigBlock],
-             v3 = Trow[j + 3 * kSigBlock];
-    trow[j] = v0;
-    trow[j + kSigBlock] = v1;
-    trow[j + 2 * kSigBlock] = v2;
-    trow[j + 3 * kSigBlock] = v3;
+  for (; j + (kStageLoads - 1) * kSigBlock < nb; j += kStageLoads * kSigBlock) {
+    VT v[kStageLoads];
+#pragma unroll
+    for (int u = 0; u < kStageLoads; u++) v[u] = Trow[j + u * kSigBlock];
+#pragma unroll
+    for (int u = 0; u < kStageLoads; u++) trow[j + u * kSigBlock] = v[u];
   }
-  for (; j < nb; j += kSigBlock) trow[j] = Trow[j];
+  if (j < nb) {
+    VT v[kStageLoads];
+#pragma unroll
+    for (int u = 0; u < kStageLoads; u++)
+      if (j + u * kSigBlock < nb) v[u] = Trow[j + u * kSigBlock];
+#pragma unroll
+    for (int u = 0; u < kStageLoads; u++)
+      if (j + u * kSigBlock < nb) trow[j + u * kSigBlock] = v[u];
+  }
 }
 
 // v, or -v when the low bit of par is set (the sign bit flipped: the same
@@ -858,27 +868,46 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_fixed_batch(const SumJob* __r
   if (threadIdx.x == 0) J.out[0] = red[0];
 }
 
-static SigmaArgs<double> sigma_args(const sbd_ctx* ctx, const double* hii, const double* T,
-                                    double* W) {
+template <typename VT>
+static SigmaArgs<VT> sigma_args(const sbd_ctx* ctx, const VT* hii, const VT* T, VT* W,
+                                int accumulate) {
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
-  return SigmaArgs<double>{ctx->d_a,  ctx->d_b,   ctx->na,    ctx->nb,   ctx->norb, hii,
-                           T,         W,          0,          A.s_off,   A.s_rec,   A.d_off,
-                           A.d_rec,   B.s_off,    B.s_rec,    B.d_off,   B.ell64,   B.ell32,
-                           B.ell_ket, B.ell_off,  B.ell_pos,  B.col_rec, ctx->d_Ea,
-                           ctx->d_Eb, (uint32_t)B.n_s,
-                           ctx->d_G,  ctx->d_eri, ctx->d_Vab};
+  return SigmaArgs<VT>{ctx->d_a,  ctx->d_b,   ctx->na,      ctx->nb,   ctx->norb, hii,
+                       T,         W,          accumulate,   A.s_off,   A.s_rec,   A.d_off,
+                       A.d_rec,   B.s_off,    B.s_rec,      B.d_off,   B.ell64,   B.ell32,
+                       B.ell_ket, B.ell_off,  B.ell_pos,    B.col_rec, ctx->d_Ea, ctx->d_Eb,
+                       (uint32_t)B.n_s, ctx->d_G, ctx->d_eri, ctx->d_Vab};
 }
 
 // rows per workgroup for one context (a property of the context alone, so a
-// job computes the same way in any batch)
-// (the most rows that fit LDS together and that the basis has: up to 4 FP64
-// rows, whose per-row state still fits the registers, or 8 FP32 rows)
+// job computes the same way in any batch): the most rows that fit LDS
+// together and that the basis has, up to 4 FP64 rows (whose per-row state
+// still fits the registers) or 8 FP32 rows
 static int sigma_rows_for(const sbd_ctx* ctx, size_t elem) {
   const size_t row_bytes = elem * ctx->nb;
   int r = elem >= 8 ? 4 : 8;
   while (r > 1 && (r * row_bytes > (size_t)ctx->lds_max || ctx->na < (uint64_t)r)) r /= 2;
   return r;
+}
+
+// Which kernel a context's sigma takes.  One rule serves single launches and
+// the serving engine's batches, so a solve computes the same way alone and in
+// a batch: the per-element kernel for sparse bases (fewer than kSparseTerms
+// off-diagonal pairs per alpha string plus per beta string), else the row
+// kernel; sbd_set_sigma_kernel can force either.
+static bool sigma_use_elems(const sbd_ctx* ctx) {
+  if (ctx->sigma_kernel == 1) return false;
+  if (ctx->sigma_kernel == 2) return true;
+  const SpinTables& A = ctx->ta;
+  const SpinTables& B = ctx->tb;
+  const double terms = (double)(A.n_s + A.n_d) / (double)ctx->na +
+                       (double)(B.n_s + B.n_d) / (double)ctx->nb;
+  return terms < kSparseTerms;
+}
+
+static bool sigma_cached(const sbd_ctx* ctx) {
+  return (ctx->ta.n_s == 0 || ctx->d_Ea) && (ctx->tb.n_s == 0 || ctx->d_Eb);
 }
 
 int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* tw_out,
@@ -888,14 +917,17 @@ int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* 
   for (int j = 0; j < n; j++) {
     const sbd_ctx* c = items[j].ctx;
     if (!c->tables_built) return SBD_ERR_STATE;
-    const bool cache = (c->ta.n_s == 0 || c->d_Ea) && (c->tb.n_s == 0 || c->d_Eb);
     const bool lds = sizeof(double) * c->nb <= (size_t)c->lds_max;
     const int r = lds ? sigma_rows_for(c, sizeof(double)) : 1;
     const uint64_t groups = (c->na + r - 1) / r;
-    if (!cache || !lds || !c->tb.ell_packed || groups > kSigMaxGrid) direct.push_back(j);
-    else byrows[r == 4 ? 2 : r == 2 ? 1 : 0].push_back(j);
+    if (sigma_use_elems(c) || !sigma_cached(c) || !lds || !c->tb.ell_packed ||
+        groups > kSigMaxGrid)
+      direct.push_back(j);
+    else
+      byrows[r == 4 ? 2 : r == 2 ? 1 : 0].push_back(j);
   }
-  // jobs outside the batched form: their own launch (same row groups and sums)
+  // jobs outside the batched form: their own launch (the same kernel, grid
+  // and sums as alone)
   for (int j : direct) {
     int rc = launch_sigma<double>(items[j].ctx, items[j].hii, items[j].T, items[j].W, 0, st,
                                   &tw_out[j]);
@@ -912,7 +944,7 @@ int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* 
     for (size_t q = 0; q < sel.size(); q++) {
       const SigmaItem& it = items[sel[q]];
       SigmaJob<double>& J = jobs[q];
-      J.p = sigma_args(it.ctx, it.hii, it.T, it.W);
+      J.p = sigma_args<double>(it.ctx, it.hii, it.T, it.W, 0);
       J.ngrp = (uint32_t)((it.ctx->na + R - 1) / R);
       J.blk0 = blk;
       blk += J.ngrp;
@@ -953,99 +985,79 @@ int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* 
   return SBD_OK;
 }
 
+// dot-product scratch of a context: n partials + one result slot
+static int sdot_reserve(sbd_ctx* ctx, size_t n) {
+  if (ctx->sdot_cap >= n + 1) return SBD_OK;
+  if (ctx->d_sdot) (void)hipFree(ctx->d_sdot);
+  ctx->d_sdot = nullptr;
+  ctx->sdot_cap = 0;
+  SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_sdot, sizeof(double) * (n + 1)));
+  ctx->sdot_cap = n + 1;
+  return SBD_OK;
+}
+
+// fixed-order sum of the np partials in part into *tw_dot (host), through
+// part[np]; synchronises the stream
+static int sdot_finish(sbd_ctx* ctx, double* part, uint64_t np, double* tw_dot, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kSumBlock), 0, st, part, np, part + np);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  SBD_HIP_TRY(ctx, hipMemcpyAsync(tw_dot, part + np, sizeof(double), hipMemcpyDeviceToHost, st));
+  SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
+  return SBD_OK;
+}
+
+// Per-element kernel (sparse bases).  part: scratch of at least
+// sigma_elems_parts(ctx) + 1 doubles when tw_dot is set.
+static uint64_t elems_tiles(const sbd_ctx* ctx) {
+  return ctx->na * ((ctx->nb + kElemTile - 1) / kElemTile);
+}
+static unsigned elems_grid(const sbd_ctx* ctx) {
+  return (unsigned)std::min<uint64_t>(elems_tiles(ctx), (uint64_t)ctx->cu_count * 32);
+}
 template <typename VT>
-int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st,
-                 double* tw_dot) {
-  if (!ctx->tables_built) {
-    ctx->last_error = "sigma called before sbd_make_helpers";
-    return SBD_ERR_STATE;
-  }
-  if (ctx->na == 0 || ctx->nb == 0) return SBD_OK;
-  // the output may not overlap the input vector or hii (the kernel's pointers
-  // are restrict-qualified, and rows of T are read after W rows are written)
-  const uintptr_t w0 = (uintptr_t)W, w1 = w0 + sizeof(VT) * ctx->N;
-  for (uintptr_t x : {(uintptr_t)T, (uintptr_t)hii}) {
-    if (x < w1 && w0 < x + sizeof(VT) * ctx->N) {
-      ctx->last_error = "sigma: Wb overlaps Wk or hii";
-      return SBD_ERR_ARG;
-    }
-  }
-  // Sparse bases (fewer than kSparseTerms off-diagonal pairs per string of
-  // each spin): the per-element kernel, same results
-  {
-    const SpinTables& A = ctx->ta;
-    const SpinTables& B = ctx->tb;
-    const double terms = (double)(A.n_s + A.n_d) / (double)ctx->na +
-                         (double)(B.n_s + B.n_d) / (double)ctx->nb;
-    const bool elems =
-        ctx->sigma_kernel == 2 || (ctx->sigma_kernel == 0 && terms < kSparseTerms);
-    if (elems) {
-      const uint64_t ntiles = ctx->na * ((ctx->nb + kElemTile - 1) / kElemTile);
-      const unsigned eg = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->cu_count * 32);
-      double* epart = nullptr;
-      if (tw_dot) {
-        const size_t need = (size_t)eg * (kElemBlock / 64) + 1;
-        if (ctx->sdot_cap < need) {
-          if (ctx->d_sdot) (void)hipFree(ctx->d_sdot);
-          ctx->d_sdot = nullptr;
-          ctx->sdot_cap = 0;
-          SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_sdot, sizeof(double) * need));
-          ctx->sdot_cap = need;
-        }
-        epart = ctx->d_sdot;
-      }
-      const SigmaArgs<VT> args{ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W,
-                               accumulate, A.s_off, A.s_rec, A.d_off, A.d_rec, B.s_off,
-                               B.s_rec, B.d_off, B.ell64, B.ell32, B.ell_ket, B.ell_off,
-                               B.ell_pos, B.col_rec, (const double*)ctx->d_Ea,
-                               (const double*)ctx->d_Eb, (uint32_t)B.n_s,
-                               (const double*)ctx->d_G, (const double*)ctx->d_eri,
-                               (const double*)ctx->d_Vab};
-      const bool ecache = (A.n_s == 0 || ctx->d_Ea) && (B.n_s == 0 || ctx->d_Eb);
-      if (ecache)
-        hipLaunchKernelGGL((k_sigma_elems<VT, true>), dim3(eg), dim3(kElemBlock), 0, st, args,
-                           B.d_rec, ntiles, epart);
-      else
-        hipLaunchKernelGGL((k_sigma_elems<VT, false>), dim3(eg), dim3(kElemBlock), 0, st, args,
-                           B.d_rec, ntiles, epart);
-      SBD_HIP_TRY(ctx, hipGetLastError());
-      if (tw_dot) {
-        const uint64_t np = (uint64_t)eg * (kElemBlock / 64);
-        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kSumBlock), 0, st, epart, np, epart + np);
-        SBD_HIP_TRY(ctx, hipGetLastError());
-        SBD_HIP_TRY(ctx,
-                    hipMemcpyAsync(tw_dot, epart + np, sizeof(double), hipMemcpyDeviceToHost, st));
-        SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
-      }
-      return SBD_OK;
-    }
-  }
+static int launch_elems(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate,
+                        hipStream_t st, double* tw_dot, double* part) {
+  const uint64_t ntiles = elems_tiles(ctx);
+  const unsigned eg = elems_grid(ctx);
+  const SigmaArgs<VT> args = sigma_args<VT>(ctx, hii, T, W, accumulate);
+  double* epart = tw_dot ? part : nullptr;
+  if (sigma_cached(ctx))
+    hipLaunchKernelGGL((k_sigma_elems<VT, true>), dim3(eg), dim3(kElemBlock), 0, st, args,
+                       ctx->tb.d_rec, ntiles, epart);
+  else
+    hipLaunchKernelGGL((k_sigma_elems<VT, false>), dim3(eg), dim3(kElemBlock), 0, st, args,
+                       ctx->tb.d_rec, ntiles, epart);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  if (tw_dot) return sdot_finish(ctx, part, (uint64_t)eg * (kElemBlock / 64), tw_dot, st);
+  return SBD_OK;
+}
+
+// Row kernel.  part: scratch of at least rows_parts(ctx) + 1 doubles when
+// tw_dot is set.
+template <typename VT>
+static unsigned rows_grid(const sbd_ctx* ctx, int* rows_out) {
   const size_t row_bytes = sizeof(VT) * ctx->nb;
   const bool lds = row_bytes <= (size_t)ctx->lds_max;
-  // rows per workgroup: as many as fit LDS together (up to 8)
   const int rows = lds ? sigma_rows_for(ctx, sizeof(VT)) : 1;
   const uint64_t groups = (ctx->na + rows - 1) / rows;
-  const unsigned grid = (unsigned)(groups < (uint64_t)kSigMaxGrid ? groups : kSigMaxGrid);
-  double* dpart = nullptr;
-  if (tw_dot) {
-    const size_t need = (size_t)grid * (kSigBlock / 64) + 1;
-    if (ctx->sdot_cap < need) {
-      if (ctx->d_sdot) (void)hipFree(ctx->d_sdot);
-      ctx->d_sdot = nullptr;
-      ctx->sdot_cap = 0;
-      SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_sdot, sizeof(double) * need));
-      ctx->sdot_cap = need;
-    }
-    dpart = ctx->d_sdot;
-  }
+  if (rows_out) *rows_out = rows;
+  return (unsigned)(groups < (uint64_t)kSigMaxGrid ? groups : kSigMaxGrid);
+}
+template <typename VT>
+static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate,
+                       hipStream_t st, double* tw_dot, double* part) {
+  const size_t row_bytes = sizeof(VT) * ctx->nb;
+  const bool lds = row_bytes <= (size_t)ctx->lds_max;
+  int rows = 1;
+  const unsigned grid = rows_grid<VT>(ctx, &rows);
+  double* dpart = tw_dot ? part : nullptr;
   const size_t shm = lds ? rows * row_bytes : 0;
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
 #define SBD_SIGMA_ARGS                                                                          \
   ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,     \
       A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell64, B.ell32, B.ell_ket, B.ell_off,      \
-      B.ell_pos, B.col_rec,                                                                     \
-      (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s,                    \
+      B.ell_pos, B.col_rec, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s, \
       (const double*)ctx->d_G, (const double*)ctx->d_eri, (const double*)ctx->d_Vab, dpart
 #define SBD_LAUNCH(LDS, CACHE, ROWS)                                                             \
   do {                                                                                           \
@@ -1057,7 +1069,7 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
                          dim3(kSigBlock), shm, st, SBD_SIGMA_ARGS);                              \
   } while (0)
   // precomputed single-excitation elements, when both caches exist
-  const bool cache = (A.n_s == 0 || ctx->d_Ea) && (B.n_s == 0 || ctx->d_Eb);
+  const bool cache = sigma_cached(ctx);
   if (lds && rows == 8) {
     if constexpr (sizeof(VT) == 4) {
       if (cache) SBD_LAUNCH(true, true, 8);
@@ -1079,14 +1091,47 @@ int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate
 #undef SBD_LAUNCH
 #undef SBD_SIGMA_ARGS
   SBD_HIP_TRY(ctx, hipGetLastError());
-  if (tw_dot) {
-    const uint64_t np = (uint64_t)grid * (kSigBlock / 64);
-    hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kSumBlock), 0, st, dpart, np, dpart + np);
-    SBD_HIP_TRY(ctx, hipGetLastError());
-    SBD_HIP_TRY(ctx, hipMemcpyAsync(tw_dot, dpart + np, sizeof(double), hipMemcpyDeviceToHost, st));
-    SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
-  }
+  if (tw_dot) return sdot_finish(ctx, part, (uint64_t)grid * (kSigBlock / 64), tw_dot, st);
   return SBD_OK;
+}
+
+// doubles of dot-product scratch a sigma of this context needs (partials and
+// the result slot)
+template <typename VT>
+static size_t sigma_scratch(const sbd_ctx* ctx) {
+  if (sigma_use_elems(ctx)) return (size_t)elems_grid(ctx) * (kElemBlock / 64) + 1;
+  return (size_t)rows_grid<VT>(ctx, nullptr) * (kSigBlock / 64) + 1;
+}
+
+template <typename VT>
+static int launch_sigma_with(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate,
+                             hipStream_t st, double* tw_dot, double* part) {
+  if (!ctx->tables_built) {
+    ctx->last_error = "sigma called before sbd_make_helpers";
+    return SBD_ERR_STATE;
+  }
+  if (ctx->na == 0 || ctx->nb == 0) return SBD_OK;
+  // the output may not overlap the input vector or hii (the kernels' pointers
+  // are restrict-qualified, and rows of T are read after W rows are written)
+  const uintptr_t w0 = (uintptr_t)W, w1 = w0 + sizeof(VT) * ctx->N;
+  for (uintptr_t x : {(uintptr_t)T, (uintptr_t)hii}) {
+    if (x < w1 && w0 < x + sizeof(VT) * ctx->N) {
+      ctx->last_error = "sigma: Wb overlaps Wk or hii";
+      return SBD_ERR_ARG;
+    }
+  }
+  if (sigma_use_elems(ctx)) return launch_elems<VT>(ctx, hii, T, W, accumulate, st, tw_dot, part);
+  return launch_rows<VT>(ctx, hii, T, W, accumulate, st, tw_dot, part);
+}
+
+template <typename VT>
+int launch_sigma(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate, hipStream_t st,
+                 double* tw_dot) {
+  if (tw_dot) {
+    int rc = sdot_reserve(ctx, sigma_scratch<VT>(ctx));
+    if (rc) return rc;
+  }
+  return launch_sigma_with<VT>(ctx, hii, T, W, accumulate, st, tw_dot, ctx->d_sdot);
 }
 
 template int launch_sigma<double>(sbd_ctx*, const double*, const double*, double*, int,

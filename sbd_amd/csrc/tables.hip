@@ -286,9 +286,11 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
     SBD_HIP_TRY(ctx, hipMalloc(&T.ell64, sizeof(EllSlot64) * ns));
     SBD_HIP_TRY(ctx, hipMalloc(&T.ell32, sizeof(EllSlot32) * ns));
     if (!T.ell_packed) SBD_HIP_TRY(ctx, hipMalloc(&T.ell_ket, sizeof(uint32_t) * ns));
-    SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell64 + T.n_slots, 0, sizeof(EllSlot64), st));
-    SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell32 + T.n_slots, 0, sizeof(EllSlot32), st));
-    if (T.ell_ket) SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell_ket + T.n_slots, 0, sizeof(uint32_t), st));
+    // all slots zero first: the lanes of the last slice past the last string
+    // are never filled and must read as zero slots (value 0, mask 0, ket 0)
+    SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell64, 0, sizeof(EllSlot64) * ns, st));
+    SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell32, 0, sizeof(EllSlot32) * ns, st));
+    if (T.ell_ket) SBD_HIP_TRY(ctx, hipMemsetAsync(T.ell_ket, 0, sizeof(uint32_t) * ns, st));
     hipLaunchKernelGGL(k_tab_ell, dim3((n + 255) / 256), dim3(256), 0, st, n, T.d_off, T.d_rec,
                        T.ell_off, T.ell_pos, T.ell_packed ? 1 : 0, T.ell64, T.ell32, T.ell_ket);
     SBD_HIP_TRY(ctx, hipGetLastError());
