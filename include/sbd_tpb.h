@@ -146,8 +146,11 @@ void sbd_ctx_destroy(sbd_ctx* ctx);
 /* Sigma kernel choice (no counterpart in the reference: a tuning knob of this
  * library).  0 (default): by sparsity -- bases with fewer than 2 off-diagonal
  * pairs per alpha string plus per beta string use the per-element kernel,
- * others the row kernel; 1: row kernel; 2: per-element kernel.  Both give the
- * same bits, and the serving engine's batches follow the same rule. */
+ * others the row kernel, whose alpha-single x beta-single terms go to a pass
+ * of their own in bases with at least 4 of them per determinant; 1: row
+ * kernel (with that pass where it applies); 2: per-element kernel; 3: row
+ * kernel with every term inline.  All give the same bits, and the serving
+ * engine's batches follow the same rule. */
 int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel);
 
 /* Dimensions: out[0] = na, out[1] = nb, out[2] = N = na*nb, out[3] = norb. */

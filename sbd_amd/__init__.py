@@ -347,8 +347,11 @@ class MultTPB:
         _check(lib().sbd_make_helpers(self._ctx), self._ctx)
 
     def set_sigma_kernel(self, kernel: str = "auto"):
-        """Sigma kernel: "auto" (by sparsity), "rows" or "elements"; same results."""
-        k = {"auto": 0, "rows": 1, "elements": 2}[kernel]
+        """Sigma kernel: "auto" (by sparsity), "rows" (row kernel; dense
+        alpha-single x beta-single terms in a pass of their own), "elements"
+        (one element per thread) or "rows_inline" (row kernel, every term
+        inline); all give the same results."""
+        k = {"auto": 0, "rows": 1, "elements": 2, "rows_inline": 3}[kernel]
         _check(lib().sbd_set_sigma_kernel(self._ctx, k), self._ctx)
 
     def table_counts(self):

@@ -550,7 +550,8 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
                   ctx->ta.ell64, ctx->ta.ell32, ctx->ta.ell_off, ctx->ta.ell_ket,
                   ctx->tb.ell64, ctx->tb.ell32, ctx->tb.ell_off, ctx->tb.ell_ket, ctx->d_sdot,
                   ctx->ta.ell_pos, ctx->tb.ell_pos, ctx->ta.col_rec, ctx->tb.col_rec,
-                  ctx->ta.s_bra,   ctx->tb.s_bra,   ctx->d_Ea,       ctx->d_Eb};
+                  ctx->ta.s_bra,   ctx->tb.s_bra,   ctx->d_Ea,       ctx->d_Eb,
+                  ctx->d_ab_sell};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_results) (void)hipHostFree(ctx->h_results);
@@ -558,7 +559,7 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
 }
 
 int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel) {
-  if (!ctx || kernel < 0 || kernel > 2) return SBD_ERR_ARG;
+  if (!ctx || kernel < 0 || kernel > 3) return SBD_ERR_ARG;
   ctx->sigma_kernel = kernel;
   return SBD_OK;
 }

@@ -97,6 +97,10 @@ __host__ __device__ inline double uniform_pm1(uint64_t h) {
   return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
+__host__ __device__ inline uint32_t parity64(uint64_t x) {
+  return (uint32_t)__builtin_popcountll(x) & 1u;
+}
+
 __host__ __device__ inline uint64_t spread32(uint64_t v) {
   v &= 0xFFFFFFFFULL;
   v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
@@ -182,6 +186,7 @@ struct SpinTables {
   ColRec* col_rec = nullptr;   // n: string and single-excitation range of each string
   uint32_t n_slices = 0;
   uint32_t* s_bra = nullptr;    // bra string of each single record
+  std::vector<uint32_t> s_off_host;  // singles offsets (host copy, n+1)
 };
 
 constexpr int kEllSlice = 64;
@@ -216,7 +221,7 @@ struct sbd_ctx {
   // Left null (on-the-fly evaluation) when they would exceed the cache budget.
   double* d_Ea = nullptr;
   double* d_Eb = nullptr;
-  uint64_t single_cache_budget = 8ULL << 30;
+  uint64_t single_cache_budget = 32ULL << 30;
   // scratch for reductions
   double* d_sdot = nullptr;  // sigma <T, W> partials (+ result slot)
   size_t sdot_cap = 0;
@@ -226,7 +231,10 @@ struct sbd_ctx {
   size_t partial_cap = 0;
   int lds_max = 65536;
   int cu_count = 256;
-  int sigma_kernel = 0;  // 0 = by sparsity, 1 = row kernel, 2 = per-element kernel
+  int sigma_kernel = 0;  // 0 = by sparsity, 1 = row kernel, 2 = per-element, 3 = rows, alpha x beta inline
+  int ab_pass = -1;      // dense alpha x beta pass: -1 not yet decided, 0 no, 1 yes
+  size_t ab_lds = 0;     // its LDS bytes per workgroup
+  uint32_t* d_ab_sell = nullptr;  // its column-major packed beta single records
   std::string last_error;
 };
 

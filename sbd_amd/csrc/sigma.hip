@@ -127,6 +127,7 @@ struct SigmaArgs {
   const double* G;    // [2][L][L][2L]
   const double* I2;   // packed ERI
   const double* Vab;  // [L][L][L][L] alpha-single x beta-single integral parts
+  int skip_ab = 0;    // leave the alpha-single x beta-single terms to k_sigma_ab
 };
 
 template <typename VT>
@@ -521,7 +522,7 @@ __device__ inline double sigma_phase2(const SigmaArgs<VT>& p, const VT* __restri
       if (!okp[j]) continue;
       // alpha single x beta single (task type 0, TwoExcite(D, ax, bx, ay, by)):
       // only rows with alpha singles (uniform) look at the column's singles
-      if (as1 > as0) {
+      if (!p.skip_ab && as1 > as0) {
         const ColRec cb = p.b_rec[col[j]];
         if (cb.s1 > cb.s0) {
           const uint64_t D0 = spread32(A[r]) | (spread32(cb.B) << 1);
@@ -658,12 +659,12 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_rows(
     const uint32_t* __restrict__ bd_ell_off_, const EllPos* __restrict__ bd_ell_pos_,
     const ColRec* __restrict__ b_rec_, const double* __restrict__ Ea_, const double* __restrict__ Eb_, uint32_t Sb_,
     const double* __restrict__ G_, const double* __restrict__ I2_,
-    const double* __restrict__ Vab_, double* __restrict__ dot_part) {
+    const double* __restrict__ Vab_, int skip_ab_, double* __restrict__ dot_part) {
   const SigmaArgs<VT> p{a_,       b_,          na_,         nb_,         L_,        hii_,
                         T_,       W_,          accumulate_, as_off_,     as_rec_,   ad_off_,
                         ad_rec_,  bs_off_,     bs_rec_,     bd_off_,     bd_ell64_, bd_ell32_,
                         bd_ell_ket_, bd_ell_off_, bd_ell_pos_, b_rec_,  Ea_,       Eb_,
-                        Sb_,      G_,          I2_,         Vab_};
+                        Sb_,      G_,          I2_,         Vab_,      skip_ab_};
   sigma_rows_body<VT, kRowLds, kCache, kRows, kPk>(p, blockIdx.x, gridDim.x, dot_part);
 }
 
@@ -682,6 +683,254 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* _
   }
   const SigmaJob<VT>& J = jobs[lo];
   sigma_rows_body<VT, true, true, kRows, true>(J.p, blockIdx.x - J.blk0, J.ngrp, J.dot_part);
+}
+
+// ---------------------------------------------------------------- dense alpha x beta
+//
+// In densely connected bases (tens of single excitations per string, as in
+// the reference's molecular inputs) the alpha-single x beta-single terms are
+// most of the sigma: each costs the row kernel three dependent global loads
+// (the beta record, a T element of the alpha ket row and an entry of the L^4
+// integral table).  k_sigma_ab adds them after the row kernel (which skips
+// them), in the same per-element order, with the data a term needs on chip:
+//
+//   * a workgroup owns a block of kAbCols columns (one per thread) and a
+//     set of rows; the beta single records of its columns are a contiguous
+//     range of the CSR and are staged in LDS once ({ket, code}, 8 bytes);
+//   * for each row ia and each alpha single (ia -> ja, orbitals xa -> ya) the
+//     L x L slice (ya xa | yb xb) of the packed ERI is staged in LDS;
+//   * the term's sign is the fermionic sign of the alpha single on D followed
+//     by the beta single on the result, which equals the reference's
+//     TwoExcite parity x order sign x (the sign with which the surviving
+//     direct or exchange integral enters): a per-row bit, a per-column bit,
+//     the record's same-spin parity bit and two prefix-parity bits of the
+//     ket alpha string A' (alpha orbitals in (xb, yb]).  The term is then
+//     that sign applied to the integral, times T[ja, jb]: the same value as
+//     the reference's, so the FP64 result is unchanged bit for bit.
+#ifndef SBD_AB_COLS
+#define SBD_AB_COLS 256
+#endif
+#ifndef SBD_AB_BATCH
+#define SBD_AB_BATCH 8
+#endif
+constexpr int kAbCols = SBD_AB_COLS;    // columns (threads) per workgroup
+constexpr int kAbKetBits = 19;          // beta strings per basis for the packed records
+constexpr int kAbBatch = SBD_AB_BATCH;  // records whose T elements are fetched together
+
+// A beta single record packed for LDS: ket | xb << 19 | yb << 25 | parity << 31.
+__device__ inline uint32_t ab_pack(const PairRec& r) {
+  return r.ket | ((uint32_t)code_x0(r.code) << kAbKetBits) |
+         ((uint32_t)code_y0(r.code) << (kAbKetBits + 6)) | ((uint32_t)code_par(r.code) << 31);
+}
+
+// Workgroup blockIdx = cb + ncb * rg: column block cb, rows rg, rg + nrg, ...
+template <typename VT>
+__global__ __launch_bounds__(kAbCols) void k_sigma_ab(SigmaArgs<VT> p, uint32_t ncb,
+                                                     uint32_t nrg) {
+  using AT = VT;
+  extern __shared__ unsigned char lds_raw[];
+  const int L = p.L;
+  double* vsl = reinterpret_cast<double*>(lds_raw);  // L x L slice
+  uint32_t* recs = reinterpret_cast<uint32_t*>(lds_raw + sizeof(double) * L * L);
+  const uint32_t nb = (uint32_t)p.nb;
+  const uint32_t cb = blockIdx.x % ncb, rg = blockIdx.x / ncb;
+  const uint32_t c0 = cb * kAbCols;
+  const uint32_t c = c0 + threadIdx.x;
+  const bool ok = c < nb;
+  const uint32_t cend = c0 + kAbCols < nb ? c0 + kAbCols : nb;
+  // this block's beta single records (a contiguous range of the CSR)
+  const uint32_t r0 = p.bs_off[c0], r1 = p.bs_off[cend];
+  for (uint32_t r = threadIdx.x; r < r1 - r0; r += kAbCols) recs[r] = ab_pack(p.bs_rec[r0 + r]);
+  const uint32_t q0 = ok ? p.bs_off[c] - r0 : 0, q1 = ok ? p.bs_off[c + 1] - r0 : 0;
+  const uint64_t B = ok ? p.b[c] : 0;
+  for (uint64_t ia = rg; ia < p.na; ia += nrg) {
+    const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
+    if (as0 == as1) continue;  // uniform: no alpha singles in this row
+    const uint64_t A = p.a[ia];
+    AT acc = (ok && q1 > q0) ? (AT)p.W[ia * nb + c] : (AT)0;
+    for (uint32_t qa = as0; qa < as1; qa++) {
+      const PairRec ra = p.as_rec[qa];
+      const int xa = code_x0(ra.code), ya = code_y0(ra.code);
+      __syncthreads();  // the previous slice is no longer read (and, first, the records are in)
+      for (int e = threadIdx.x; e < L * L; e += kAbCols)
+        vsl[e] = p.I2[eri_index(ya, xa, e % L, e / L)];  // (ya xa | yb xb) at e = xb L + yb
+      __syncthreads();
+      if (q1 == q0) continue;
+      const int lo = xa < ya ? xa : ya, hi = xa < ya ? ya : xa;
+      const uint64_t PA1 = prefix_parity(A ^ (1ULL << xa) ^ (1ULL << ya));
+      const uint32_t s_col =
+          parity64(A & bits_range(lo + 1, hi)) ^ parity64(B & bits_range(lo, hi));
+      const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
+      for (uint32_t k0 = q0; k0 < q1; k0 += kAbBatch) {
+        uint32_t rb[kAbBatch];
+        VT tv[kAbBatch];
+#pragma unroll
+        for (int u = 0; u < kAbBatch; u++) {
+          rb[u] = recs[k0 + u < q1 ? k0 + u : q1 - 1];
+          tv[u] = Tk[rb[u] & ((1u << kAbKetBits) - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < kAbBatch; u++) {
+          const int xb = (rb[u] >> kAbKetBits) & 63, yb = (rb[u] >> (kAbKetBits + 6)) & 63;
+          const uint32_t sg = s_col ^ (rb[u] >> 31) ^ (uint32_t)((PA1 >> xb) ^ (PA1 >> yb));
+          const double e = flip_sign(vsl[xb * L + yb], sg & 1u);
+          const AT nv = acc + (AT)e * (AT)tv[u];
+          acc = (k0 + u < q1) ? nv : acc;
+        }
+      }
+    }
+    if (ok && q1 > q0) p.W[ia * nb + c] = (VT)acc;
+  }
+}
+
+// Row-wise form of the dense pass (bases of up to kAbRowBlock * 8 beta
+// strings): one 1024-thread workgroup per bra row (grid-stride over rows),
+// thread t owning columns t + 1024 j (j < C) with their sums in registers
+// across all of the row's alpha singles.  For each alpha single the ket row
+// T[ja, :] and the integral slice are staged in LDS; with two buffers (when
+// they fit) the next ones are loaded into registers while the current ones are
+// used and stored after, one barrier per alpha single.  The beta records come
+// from a column-major packed copy (record k of column c at k nb + c), so a
+// wavefront's record loads are coalesced; every gather of the inner loop is
+// from LDS.  Same terms, same order, same operations as k_sigma_ab.
+constexpr int kAbRowBlock = 1024;
+constexpr int kAbRowBatch = 8;
+
+template <typename VT, int C, bool kDbuf>
+__global__ __launch_bounds__(kAbRowBlock) void k_sigma_ab_rows(SigmaArgs<VT> p,
+                                                              const uint32_t* __restrict__ sell) {
+  using AT = VT;
+  extern __shared__ unsigned char lds_raw[];
+  const int L = p.L;
+  const int LL = L * L;
+  const uint32_t nb = (uint32_t)p.nb;
+  const size_t row_bytes = ((size_t)nb * sizeof(VT) + 15) & ~(size_t)15;
+  const size_t buf_bytes = row_bytes + sizeof(double) * LL;
+  auto trow = [&](int b) { return reinterpret_cast<VT*>(lds_raw + b * buf_bytes); };
+  auto vslc = [&](int b) { return reinterpret_cast<double*>(lds_raw + b * buf_bytes + row_bytes); };
+  constexpr int kV = 4;  // slice elements a thread stages (L*L <= 4096)
+  // this thread's columns: record counts and strings (the same for every row)
+  uint32_t cnt[C];
+  uint64_t B[C];
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const uint32_t c = threadIdx.x + j * kAbRowBlock;
+    cnt[j] = c < nb ? p.bs_off[c + 1] - p.bs_off[c] : 0;
+    B[j] = c < nb ? p.b[c] : 0;
+  }
+  auto stage = [&](int b, uint32_t qa) {  // synchronous staging of alpha single qa
+    const PairRec ra = p.as_rec[qa];
+    const int xa = code_x0(ra.code), ya = code_y0(ra.code);
+    const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
+    VT* tr = trow(b);
+    double* vs = vslc(b);
+    VT v[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      const uint32_t c = threadIdx.x + j * kAbRowBlock;
+      v[j] = c < nb ? Tk[c] : (VT)0;
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      const uint32_t c = threadIdx.x + j * kAbRowBlock;
+      if (c < nb) tr[c] = v[j];
+    }
+    for (int e = threadIdx.x; e < LL; e += kAbRowBlock) vs[e] = p.I2[eri_index(ya, xa, e % L, e / L)];
+  };
+  for (uint64_t ia = blockIdx.x; ia < p.na; ia += gridDim.x) {
+    const uint32_t as0 = p.as_off[ia], as1 = p.as_off[ia + 1];
+    if (as0 == as1) continue;  // uniform: no alpha singles in this row
+    const uint64_t A = p.a[ia];
+    AT acc[C];
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      acc[j] = cnt[j] ? (AT)p.W[ia * nb + threadIdx.x + j * kAbRowBlock] : (AT)0;
+    __syncthreads();  // the buffers of the previous row are no longer read
+    stage(0, as0);
+    __syncthreads();
+    for (uint32_t qa = as0; qa < as1; qa++) {
+      const int cur = kDbuf ? (int)((qa - as0) & 1) : 0;
+      // next alpha single's ket row and slice, into registers
+      VT nt[kDbuf ? C : 1];
+      double nv[kDbuf ? kV : 1];
+      const bool more = kDbuf && qa + 1 < as1;  // uniform
+      if (more) {
+        const PairRec rn = p.as_rec[qa + 1];
+        const int xn = code_x0(rn.code), yn = code_y0(rn.code);
+        const VT* __restrict__ Tn = p.T + (uint64_t)rn.ket * nb;
+#pragma unroll
+        for (int j = 0; j < (kDbuf ? C : 1); j++) {
+          const uint32_t c = threadIdx.x + j * kAbRowBlock;
+          nt[j] = c < nb ? Tn[c] : (VT)0;
+        }
+#pragma unroll
+        for (int i = 0; i < (kDbuf ? kV : 1); i++) {
+          const int e = threadIdx.x + i * kAbRowBlock;
+          nv[i] = e < LL ? p.I2[eri_index(yn, xn, e % L, e / L)] : 0.0;
+        }
+      }
+      const PairRec ra = p.as_rec[qa];
+      const int xa = code_x0(ra.code), ya = code_y0(ra.code);
+      const int lo = xa < ya ? xa : ya, hi = xa < ya ? ya : xa;
+      const uint64_t PA1 = prefix_parity(A ^ (1ULL << xa) ^ (1ULL << ya));
+      const uint32_t sA = parity64(A & bits_range(lo + 1, hi));
+      const VT* tr = trow(cur);
+      const double* vs = vslc(cur);
+#pragma unroll
+      for (int j = 0; j < C; j++) {
+        if (!cnt[j]) continue;
+        const uint32_t c = threadIdx.x + j * kAbRowBlock;
+        const uint32_t s_col = sA ^ parity64(B[j] & bits_range(lo, hi));
+        for (uint32_t k0 = 0; k0 < cnt[j]; k0 += kAbRowBatch) {
+          uint32_t rb[kAbRowBatch];
+#pragma unroll
+          for (int u = 0; u < kAbRowBatch; u++)
+            rb[u] = sell[(uint64_t)(k0 + u < cnt[j] ? k0 + u : cnt[j] - 1) * nb + c];
+#pragma unroll
+          for (int u = 0; u < kAbRowBatch; u++) {
+            const int xb = (rb[u] >> kAbKetBits) & 63, yb = (rb[u] >> (kAbKetBits + 6)) & 63;
+            const uint32_t sg = s_col ^ (rb[u] >> 31) ^ (uint32_t)((PA1 >> xb) ^ (PA1 >> yb));
+            const double e = flip_sign(vs[xb * L + yb], sg & 1u);
+            const AT nv2 = acc[j] + (AT)e * (AT)tr[rb[u] & ((1u << kAbKetBits) - 1)];
+            acc[j] = (k0 + u < cnt[j]) ? nv2 : acc[j];
+          }
+        }
+      }
+      if (kDbuf) {
+        if (more) {
+          VT* tn = trow(cur ^ 1);
+          double* vn = vslc(cur ^ 1);
+#pragma unroll
+          for (int j = 0; j < (kDbuf ? C : 1); j++) {
+            const uint32_t c = threadIdx.x + j * kAbRowBlock;
+            if (c < nb) tn[c] = nt[j];
+          }
+#pragma unroll
+          for (int i = 0; i < (kDbuf ? kV : 1); i++) {
+            const int e = threadIdx.x + i * kAbRowBlock;
+            if (e < LL) vn[e] = nv[i];
+          }
+        }
+        __syncthreads();
+      } else if (qa + 1 < as1) {
+        __syncthreads();
+        stage(0, qa + 1);
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      if (cnt[j]) p.W[ia * nb + threadIdx.x + j * kAbRowBlock] = (VT)acc[j];
+  }
+}
+
+// Column-major packed copy of the beta single records for k_sigma_ab_rows:
+// record k of column c (k < its count) at k nb + c.
+__global__ void k_ab_sell(uint32_t nb, const uint32_t* __restrict__ off,
+                          const PairRec* __restrict__ rec, uint32_t* __restrict__ sell) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  for (uint32_t k = 0; k < off[c + 1] - off[c]; k++) sell[(uint64_t)k * nb + c] = ab_pack(rec[off[c] + k]);
 }
 
 // Sparse bases (a few off-diagonal terms per determinant): one element per
@@ -897,7 +1146,7 @@ static int sigma_rows_for(const sbd_ctx* ctx, size_t elem) {
 // off-diagonal pairs per alpha string plus per beta string), else the row
 // kernel; sbd_set_sigma_kernel can force either.
 static bool sigma_use_elems(const sbd_ctx* ctx) {
-  if (ctx->sigma_kernel == 1) return false;
+  if (ctx->sigma_kernel == 1 || ctx->sigma_kernel == 3) return false;
   if (ctx->sigma_kernel == 2) return true;
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
@@ -910,18 +1159,76 @@ static bool sigma_cached(const sbd_ctx* ctx) {
   return (ctx->ta.n_s == 0 || ctx->d_Ea) && (ctx->tb.n_s == 0 || ctx->d_Eb);
 }
 
+// The row kernel leaves the alpha-single x beta-single terms to k_sigma_ab
+// when the basis has at least kAbDense of them per determinant, its beta
+// strings fit the packed records, and a column block's records and the
+// integral slice fit LDS (decided once per context; knob 3 keeps them
+// inline).
+constexpr double kAbDense = 4.0;
+static bool sigma_ab_pass(sbd_ctx* ctx) {
+  if (ctx->sigma_kernel == 3) return false;
+  if (ctx->ab_pass < 0) {
+    ctx->ab_pass = 0;
+    const SpinTables& A = ctx->ta;
+    const SpinTables& B = ctx->tb;
+    const double terms = ((double)A.n_s / (double)ctx->na) * ((double)B.n_s / (double)ctx->nb);
+    if (terms >= kAbDense && ctx->nb <= (1ULL << kAbKetBits) &&
+        B.s_off_host.size() == ctx->nb + 1) {
+      uint32_t mx = 0;
+      for (uint64_t c0 = 0; c0 < ctx->nb; c0 += kAbCols) {
+        const uint64_t c1 = std::min<uint64_t>(ctx->nb, c0 + kAbCols);
+        mx = std::max(mx, B.s_off_host[c1] - B.s_off_host[c0]);
+      }
+      const size_t lds = sizeof(double) * ctx->norb * ctx->norb + sizeof(uint32_t) * (size_t)mx;
+      if (lds <= (size_t)ctx->lds_max) {
+        ctx->ab_pass = 1;
+        ctx->ab_lds = lds;
+      }
+    }
+  }
+  return ctx->ab_pass == 1;
+}
+
+// Row-wise form of the dense pass (k_sigma_ab_rows) for a vector of element
+// size elem: columns per thread (0: the basis is too wide for it), whether
+// two LDS buffers fit, and their bytes.  Builds the column-major record copy
+// on first use.
+static int sigma_ab_rows_cfg(sbd_ctx* ctx, size_t elem, int* C, bool* dbuf, size_t* lds,
+                             hipStream_t st) {
+  *C = 0;
+  const uint64_t per = (ctx->nb + kAbRowBlock - 1) / kAbRowBlock;
+  if (per > 8) return SBD_OK;  // wider rows take the column-block form
+  *C = per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : 8;
+  const size_t row_bytes = (elem * ctx->nb + 15) & ~(size_t)15;
+  const size_t buf = row_bytes + sizeof(double) * ctx->norb * ctx->norb;
+  *dbuf = 2 * buf <= (size_t)ctx->lds_max;
+  *lds = *dbuf ? 2 * buf : buf;
+  if (*lds > (size_t)ctx->lds_max) {
+    *C = 0;
+    return SBD_OK;
+  }
+  if (!ctx->d_ab_sell) {
+    const size_t n = (size_t)std::max<uint32_t>(ctx->tb.max_s, 1) * ctx->nb;
+    SBD_HIP_TRY(ctx, hipMalloc(&ctx->d_ab_sell, sizeof(uint32_t) * n));
+    hipLaunchKernelGGL(k_ab_sell, dim3((unsigned)((ctx->nb + 255) / 256)), dim3(256), 0, st,
+                       (uint32_t)ctx->nb, ctx->tb.s_off, ctx->tb.s_rec, ctx->d_ab_sell);
+    SBD_HIP_TRY(ctx, hipGetLastError());
+  }
+  return SBD_OK;
+}
+
 int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* tw_out,
                        hipStream_t st) {
   if (n < 1) return SBD_OK;
   std::vector<int> direct, byrows[3];  // rows 1, 2, 4
   for (int j = 0; j < n; j++) {
-    const sbd_ctx* c = items[j].ctx;
+    sbd_ctx* c = items[j].ctx;
     if (!c->tables_built) return SBD_ERR_STATE;
     const bool lds = sizeof(double) * c->nb <= (size_t)c->lds_max;
     const int r = lds ? sigma_rows_for(c, sizeof(double)) : 1;
     const uint64_t groups = (c->na + r - 1) / r;
-    if (sigma_use_elems(c) || !sigma_cached(c) || !lds || !c->tb.ell_packed ||
-        groups > kSigMaxGrid)
+    if (sigma_use_elems(c) || sigma_ab_pass(c) || !sigma_cached(c) || !lds ||
+        !c->tb.ell_packed || groups > kSigMaxGrid)
       direct.push_back(j);
     else
       byrows[r == 4 ? 2 : r == 2 ? 1 : 0].push_back(j);
@@ -1050,7 +1357,9 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
   const bool lds = row_bytes <= (size_t)ctx->lds_max;
   int rows = 1;
   const unsigned grid = rows_grid<VT>(ctx, &rows);
-  double* dpart = tw_dot ? part : nullptr;
+  const int skip_ab = sigma_ab_pass(ctx) ? 1 : 0;
+  // with the dense pass the row kernel's W is not final: <T, W> comes after it
+  double* dpart = (tw_dot && !skip_ab) ? part : nullptr;
   const size_t shm = lds ? rows * row_bytes : 0;
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
@@ -1058,7 +1367,7 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
   ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,     \
       A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell64, B.ell32, B.ell_ket, B.ell_off,      \
       B.ell_pos, B.col_rec, (const double*)ctx->d_Ea, (const double*)ctx->d_Eb, (uint32_t)B.n_s, \
-      (const double*)ctx->d_G, (const double*)ctx->d_eri, (const double*)ctx->d_Vab, dpart
+      (const double*)ctx->d_G, (const double*)ctx->d_eri, (const double*)ctx->d_Vab, skip_ab, dpart
 #define SBD_LAUNCH(LDS, CACHE, ROWS)                                                             \
   do {                                                                                           \
     if (B.ell_packed)                                                                            \
@@ -1091,6 +1400,47 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
 #undef SBD_LAUNCH
 #undef SBD_SIGMA_ARGS
   SBD_HIP_TRY(ctx, hipGetLastError());
+  if (skip_ab) {
+    const SigmaArgs<VT> args = sigma_args<VT>(ctx, hii, T, W, 1);
+    int C = 0;
+    bool dbuf = false;
+    size_t lds_rows = 0;
+    int rc = sigma_ab_rows_cfg(ctx, sizeof(VT), &C, &dbuf, &lds_rows, st);
+    if (rc) return rc;
+    if (C) {
+      // one workgroup per CU over the rows
+      const unsigned g = (unsigned)std::min<uint64_t>(ctx->na, (uint64_t)ctx->cu_count);
+#define SBD_AB_ROWS(CC)                                                                           \
+  do {                                                                                            \
+    if (dbuf)                                                                                     \
+      hipLaunchKernelGGL((k_sigma_ab_rows<VT, CC, true>), dim3(g), dim3(kAbRowBlock), lds_rows, st, \
+                         args, (const uint32_t*)ctx->d_ab_sell);                                  \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_sigma_ab_rows<VT, CC, false>), dim3(g), dim3(kAbRowBlock), lds_rows,  \
+                         st, args, (const uint32_t*)ctx->d_ab_sell);                              \
+  } while (0)
+      switch (C) {
+        case 1: SBD_AB_ROWS(1); break;
+        case 2: SBD_AB_ROWS(2); break;
+        case 4: SBD_AB_ROWS(4); break;
+        default: SBD_AB_ROWS(8); break;
+      }
+#undef SBD_AB_ROWS
+    } else {
+      const uint32_t ncb = (uint32_t)((ctx->nb + kAbCols - 1) / kAbCols);
+      // about eight workgroups per CU, each over a share of the rows
+      const uint64_t want = ((uint64_t)ctx->cu_count * 8 + ncb - 1) / ncb;
+      const uint32_t nrg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, ctx->na));
+      hipLaunchKernelGGL(k_sigma_ab<VT>, dim3(ncb * nrg), dim3(kAbCols), ctx->ab_lds, st, args,
+                         ncb, nrg);
+    }
+    SBD_HIP_TRY(ctx, hipGetLastError());
+    if (tw_dot) {
+      const VT* X[1] = {T};
+      return vec_multidot<VT>(ctx, X, 1, W, ctx->N, tw_dot, st);
+    }
+    return SBD_OK;
+  }
   if (tw_dot) return sdot_finish(ctx, part, (uint64_t)grid * (kSigBlock / 64), tw_dot, st);
   return SBD_OK;
 }
@@ -1175,7 +1525,17 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
       (const void*)k_sigma_rows<float, true, false, 8, false>,
       (const void*)k_sigma_batch<double, 1>,
       (const void*)k_sigma_batch<double, 2>,
-      (const void*)k_sigma_batch<double, 4>};
+      (const void*)k_sigma_batch<double, 4>,
+      (const void*)k_sigma_ab<double>,
+      (const void*)k_sigma_ab<float>,
+#define SBD_AB_FNS(VT)                                                                            \
+  (const void*)k_sigma_ab_rows<VT, 1, true>, (const void*)k_sigma_ab_rows<VT, 2, true>,           \
+      (const void*)k_sigma_ab_rows<VT, 4, true>, (const void*)k_sigma_ab_rows<VT, 8, true>,       \
+      (const void*)k_sigma_ab_rows<VT, 1, false>, (const void*)k_sigma_ab_rows<VT, 2, false>,     \
+      (const void*)k_sigma_ab_rows<VT, 4, false>, (const void*)k_sigma_ab_rows<VT, 8, false>
+      SBD_AB_FNS(double), SBD_AB_FNS(float)
+#undef SBD_AB_FNS
+  };
   for (const void* f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipGetLastError();

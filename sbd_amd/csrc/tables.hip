@@ -241,6 +241,8 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
   (void)hipFree(dc);
   T.n_s = tot_s;
   T.n_d = tot_d;
+  T.s_off_host.assign(n + 1, 0);
+  for (uint32_t i = 0; i < n; i++) T.s_off_host[i + 1] = T.s_off_host[i] + hs[i];
   T.max_s = mx_s;
   T.max_d = mx_d;
   // strings with at least one single excitation

@@ -137,10 +137,11 @@ def test_tables_hdiag_sigma_bitexact(torch, name, factory):
     W32 = m.sigma(h32, Wk.float())
     err = np.abs(W32.cpu().numpy().astype(np.float64) - s_ref).max() / np.abs(s_ref).max()
     assert err < 2e-5, err
-    say(f"case {name}: both sigma kernels")
-    # the row kernel and the per-element kernel give the same bits, FP64
-    # (overwrite and accumulate) and FP32
-    for kernel in ("rows", "elements"):
+    say(f"case {name}: every sigma kernel")
+    # the row kernel (with the dense alpha x beta pass where it applies), the
+    # per-element kernel and the row kernel with every term inline give the
+    # same bits, FP64 (overwrite and accumulate) and FP32
+    for kernel in ("rows", "elements", "rows_inline"):
         m.set_sigma_kernel(kernel)
         assert np.array_equal(m.sigma(hii, Wk).cpu().numpy(), s_ref), kernel
         Wacc2 = torch.tensor(W0, device="cuda")
