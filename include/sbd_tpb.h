@@ -113,6 +113,9 @@ typedef struct {
   double t_davidson;      /* seconds: Davidson                                   */
   double t_sigma;         /* seconds inside sigma kernels during Davidson        */
   int steps_fp32;         /* precision = 1: steps taken in FP32                  */
+  int nb_used;            /* subspace size of the FP64 phase (max_nb, or fewer
+                             when 2 nb + 2 vectors would not fit free memory)    */
+  int nb_used_fp32;       /* the same for the FP32 phase (precision = 1)         */
 } sbd_report;
 
 const char* sbd_status_string(int status);

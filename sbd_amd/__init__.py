@@ -59,7 +59,8 @@ class Options(C.Structure):
 class Report(C.Structure):
     _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("energy", C.c_double), ("t_tables", C.c_double), ("t_hdiag", C.c_double),
-                ("t_davidson", C.c_double), ("t_sigma", C.c_double), ("steps_fp32", C.c_int)]
+                ("t_davidson", C.c_double), ("t_sigma", C.c_double), ("steps_fp32", C.c_int),
+                ("nb_used", C.c_int), ("nb_used_fp32", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -239,9 +239,23 @@ struct DevBuf {
   }
 };
 
+// Largest subspace size up to nb whose Davidson vectors fit the free device
+// memory: 2 nb vectors besides W and the diagonal (C[1..nb-1], HC[0..nb-1] and
+// the spare; C[0] is W itself), with 2% of the memory left for the rest.
+static int fit_subspace(int nb, uint64_t vec_bytes) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess || vec_bytes == 0) return nb;
+  const double usable = (double)fr - 0.02 * (double)tot;
+  const int fit = (int)std::floor(usable / (double)vec_bytes / 2.0);
+  return std::max(1, std::min(nb, fit));
+}
+
 // Single-root Davidson on device vectors (davidson.h:707-970), driven pass
 // by pass through DavidsonJob (davidson_job.hpp) with the fused vector passes
-// of vecops.hip.  W: device, in/out (normalised Ritz vector on exit).
+// of vecops.hip.  W: device, in/out (normalised Ritz vector on exit); it is
+// the first basis vector itself, so a solve holds 2 nb + 2 vectors with the
+// diagonal.  The subspace is cut below nb when those would not fit the free
+// device memory (rep->nb_used tells the size taken).
 template <typename VT>
 int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double eps,
                   double max_time, int verbose, double* energy, std::vector<double>* hist,
@@ -250,23 +264,33 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
   hipStream_t st = nullptr;
   if (nb < 1 || nb > kMaxVecs) return set_err(ctx, SBD_ERR_ARG, "max_nb must be in [1, 40]");
   if (max_it < 1) return set_err(ctx, SBD_ERR_ARG, "max_it must be >= 1");
+  nb = fit_subspace(nb, sizeof(VT) * N);
+  // C[0] is W; the other vectors are allocated when a pass first needs them
+  // (a solve that converges in a few steps holds a few vectors) and freed on
+  // every return.  Restarts only swap the roles of C[0] and the spare, so the
+  // allocated buffers (all but W's) stay owned by Cb, HCb and spare.
   std::vector<DevBuf<VT>> Cb(nb), HCb(nb);
   DevBuf<VT> spare;
-  for (int k = 0; k < nb; k++) {
-    if (hipMalloc(&Cb[k].p, sizeof(VT) * N) != hipSuccess ||
-        hipMalloc(&HCb[k].p, sizeof(VT) * N) != hipSuccess)
-      return set_err(ctx, SBD_ERR_MEMORY, "device allocation of Davidson vectors failed");
+  std::vector<VT*> C(nb, nullptr), HC(nb, nullptr);
+  C[0] = W;
+  if (rep) {
+    if (sizeof(VT) == 4) rep->nb_used_fp32 = nb;
+    else rep->nb_used = nb;
   }
-  if (hipMalloc(&spare.p, sizeof(VT) * N) != hipSuccess)
-    return set_err(ctx, SBD_ERR_MEMORY, "device allocation of Davidson vectors failed");
-  std::vector<VT*> C(nb), HC(nb);
-  for (int k = 0; k < nb; k++) {
-    C[k] = Cb[k].p;
-    HC[k] = HCb[k].p;
-  }
-  SBD_HIP_TRY(ctx, hipMemcpyAsync(C[0], W, sizeof(VT) * N, hipMemcpyDeviceToDevice, st));
   DavidsonJob<VT> job;
-  job.init(C, HC, spare.p, dii, N, max_it, eps);
+  job.init(C, HC, nullptr, dii, N, max_it, eps);
+  // the buffer behind a pointer the next pass writes, allocated on first use
+  auto need = [&](VT*& slot, DevBuf<VT>& owner) -> int {
+    if (slot) return SBD_OK;
+    if (hipMalloc(&owner.p, sizeof(VT) * N) != hipSuccess)
+      return set_err(ctx, SBD_ERR_MEMORY, "device allocation of Davidson vectors failed");
+    slot = owner.p;
+    return SBD_OK;
+  };
+  auto need_t = [&]() -> int {  // the Ritz pass writes C[ib + 1] or, last, the spare
+    const int i = job.ib + 1;
+    return i < nb ? need(job.C[i], Cb[i]) : need(job.spare, spare);
+  };
   int rc = 0;
   double t_sigma = 0.0, step_t0 = now_s();
   std::vector<double> step_times;
@@ -281,6 +305,8 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
       case DavidsonJob<VT>::kSigma: {
         step_t0 = now_s();
         double td = 0.0;
+        rc = need(job.HC[job.ib], HCb[job.ib]);
+        if (rc) break;
         SBD_HIP_TRY(ctx, hipEventRecord(ev0, st));
         rc = launch_sigma<VT>(ctx, dii, job.sigma_in(), job.sigma_out(), 0, st, &td);
         if (rc) break;
@@ -294,6 +320,8 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
       }
       case DavidsonJob<VT>::kRitz: {
         const int k = job.k();
+        rc = need_t();
+        if (rc) break;
         const auto Cc = ptrs(job.C, k), HCc = ptrs(job.HC, k);
         rc = vec_ritz_precond<VT>(ctx, Cc.data(), HCc.data(), k, job.ritz_u(), job.E0, dii, job.t(),
                                   N, job.ritz_sums(), st);
@@ -332,6 +360,8 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
       }
       case DavidsonJob<VT>::kLincomb: {
         const int k = job.lincomb_k();
+        rc = need(job.spare, spare);
+        if (rc) break;
         const auto Cc = ptrs(job.C, k);
         double n2 = 0.0;
         rc = vec_lincomb<VT>(ctx, Cc.data(), k, job.ritz_u(), 1.0, job.lincomb_out(), N, &n2, st);
@@ -361,15 +391,19 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
       }
     }
   }
-  // the buffers may have been rotated by restarts: keep ownership consistent
-  for (int k = 0; k < nb; k++) Cb[k].p = job.C[k];
-  spare.p = job.spare;
   if (!rc) {
-    // W = normalised Ritz vector of the last step
+    // W = normalised Ritz vector of the last step, formed in the spare (W's
+    // buffer may be one of the basis vectors it is formed from) and copied
+    // into W's buffer when they differ
     double n2 = 0.0;
     const auto Cc = ptrs(job.C, job.last_k);
-    rc = vec_lincomb<VT>(ctx, Cc.data(), job.last_k, job.ritz_u(), 1.0, W, N, &n2, st);
-    if (!rc) rc = vec_scale<VT>(ctx, W, 1.0 / std::sqrt(n2), N, st);
+    rc = need(job.spare, spare);
+    VT* out = job.spare;
+    if (!rc) rc = vec_lincomb<VT>(ctx, Cc.data(), job.last_k, job.ritz_u(), 1.0, out, N, &n2, st);
+    if (!rc) rc = vec_scale<VT>(ctx, out, 1.0 / std::sqrt(n2), N, st);
+    if (!rc && out != W && hipMemcpyAsync(W, out, sizeof(VT) * N, hipMemcpyDeviceToDevice, st) !=
+                               hipSuccess)
+      rc = SBD_ERR_DEVICE;
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = SBD_ERR_DEVICE;
   }
   (void)hipEventDestroy(ev0);
@@ -385,6 +419,50 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
   return rc;
 }
 
+}  // namespace
+
+namespace {
+// W[0, N) = the start vector (BasisInitVector, davidson.h:144-184) in the
+// element type of W, built on the device: the caller's (host, FP64; a loaded
+// wavefunction, sbdiag.h:339-347), else e_0 (init 0) or the seeded Randomize
+// vector (init 1; seed 0 draws a seed from the clock, as the reference does).
+template <typename VT>
+int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT* W) {
+  const uint64_t N = ctx->N;
+  int rc = SBD_OK;
+  if (W_init) {
+    if (sizeof(VT) == 8) {
+      SBD_HIP_TRY(ctx, hipMemcpy(W, W_init, 8 * N, hipMemcpyHostToDevice));
+    } else {
+      // through an FP64 staging chunk, converted in place
+      const uint64_t chunk = std::min<uint64_t>(N, 1ULL << 25);
+      DevBuf<double> stage;
+      if (hipMalloc(&stage.p, 8 * chunk) != hipSuccess)
+        return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
+      for (uint64_t i0 = 0; i0 < N && !rc; i0 += chunk) {
+        const uint64_t n = std::min(chunk, N - i0);
+        SBD_HIP_TRY(ctx, hipMemcpy(stage.p, W_init + i0, 8 * n, hipMemcpyHostToDevice));
+        rc = vec_convert<double, VT>(ctx, stage.p, W + i0, n, nullptr);
+        if (!rc && hipDeviceSynchronize() != hipSuccess) rc = SBD_ERR_DEVICE;
+      }
+    }
+  } else if (opt->init == 0) {
+    rc = vec_fill<VT>(ctx, W, N, 0.0, nullptr);
+    const VT one = (VT)1;
+    if (!rc) SBD_HIP_TRY(ctx, hipMemcpy(W, &one, sizeof(VT), hipMemcpyHostToDevice));
+  } else {
+    uint64_t seed = opt->seed;
+    if (seed == 0) {
+      const uint64_t ns = (uint64_t)std::chrono::high_resolution_clock::now()
+                              .time_since_epoch()
+                              .count();
+      seed = elem_hash(ns, 0);  // sbd_time_seed at rank 0
+    }
+    rc = vec_random<VT>(ctx, W, N, seed, nullptr);
+  }
+  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = SBD_ERR_DEVICE;
+  return rc;
+}
 }  // namespace
 
 extern "C" {
@@ -753,43 +831,42 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
   if (rc) return rc;
   SBD_HIP_TRY(ctx, hipDeviceSynchronize());
   R->t_tables = now_s() - t0;
-  // initial vector (BasisInitVector, davidson.h:144-184), or the caller's
-  // (a loaded wavefunction, sbdiag.h:339-347)
-  std::vector<double> W0(N, 0.0);
-  if (W_init) {
-    std::memcpy(W0.data(), W_init, 8 * N);
-  } else if (opt->init == 0) {
-    W0[0] = 1.0;
-  } else {
-    randomize(W0.data(), N, opt->seed);
-  }
-  DevBuf<double> hii, W, HW;
-  if (hipMalloc(&hii.p, 8 * N) != hipSuccess || hipMalloc(&W.p, 8 * N) != hipSuccess)
-    return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-  SBD_HIP_TRY(ctx, hipMemcpy(W.p, W0.data(), 8 * N, hipMemcpyHostToDevice));
-  t0 = now_s();
-  rc = launch_hdiag(ctx, hii.p, nullptr);
-  if (rc) return rc;
-  SBD_HIP_TRY(ctx, hipDeviceSynchronize());
-  R->t_hdiag = now_s() - t0;
-  t0 = now_s();
+  // Device memory is held to what each phase needs: the FP32 phase holds its
+  // vectors and FP32 diagonal only, and frees them (but W) before the FP64
+  // diagonal is made; the final <W, H W> takes its H W buffer once the
+  // subspace vectors are gone.
+  DevBuf<double> W, hii;
   double E = 0.0;
+  t0 = now_s();
+  const double t_solve0 = t0;
   if (opt->precision == 1) {
     // FP32 Davidson (FP32 vectors and sigma; FP64 reductions, Ritz and GS),
     // then the FP64 correction sweep from its vector.
-    DevBuf<float> hii32, W32;
-    if (hipMalloc(&hii32.p, 4 * N) != hipSuccess || hipMalloc(&W32.p, 4 * N) != hipSuccess)
+    DevBuf<float> W32;
+    {
+      DevBuf<float> hii32;
+      if (hipMalloc(&hii32.p, 4 * N) != hipSuccess || hipMalloc(&W32.p, 4 * N) != hipSuccess)
+        return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
+      rc = start_vector<float>(ctx, opt, W_init, W32.p);
+      if (rc) return rc;
+      const double th = now_s();
+      rc = launch_hdiag_f32(ctx, hii32.p, nullptr);
+      if (rc) return rc;
+      SBD_HIP_TRY(ctx, hipDeviceSynchronize());
+      R->t_hdiag += now_s() - th;
+      const double e32 = std::max(opt->fp32_eps, opt->eps);
+      rc = davidson_core<float>(ctx, hii32.p, W32.p, opt->max_it, opt->max_nb, e32, opt->max_time,
+                                opt->verbose, &E, H, R, t_solve0);
+      if (rc) return rc;
+      R->steps_fp32 = R->steps;
+    }
+    if (hipMalloc(&W.p, 8 * N) != hipSuccess)
       return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-    rc = vec_convert<double, float>(ctx, hii.p, hii32.p, N, nullptr);
-    if (!rc) rc = vec_convert<double, float>(ctx, W.p, W32.p, N, nullptr);
-    if (rc) return rc;
-    const double e32 = std::max(opt->fp32_eps, opt->eps);
-    rc = davidson_core<float>(ctx, hii32.p, W32.p, opt->max_it, opt->max_nb, e32, opt->max_time,
-                              opt->verbose, &E, H, R, t0);
-    if (rc) return rc;
-    R->steps_fp32 = R->steps;
     rc = vec_convert<float, double>(ctx, W32.p, W.p, N, nullptr);
     if (rc) return rc;
+    SBD_HIP_TRY(ctx, hipDeviceSynchronize());
+    (void)hipFree(W32.p);
+    W32.p = nullptr;
     // renormalise in FP64: the FP32 vector's norm is 1 only to FP32 accuracy,
     // and the Davidson start vector is taken as normalised
     const double* Wc[1] = {W.p};
@@ -797,31 +874,43 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
     rc = vec_multidot<double>(ctx, Wc, 1, W.p, N, &n2, nullptr);
     if (!rc && n2 > 0.0) rc = vec_scale<double>(ctx, W.p, 1.0 / std::sqrt(n2), N, nullptr);
     if (rc) return rc;
+  } else {
+    if (hipMalloc(&W.p, 8 * N) != hipSuccess)
+      return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
+    rc = start_vector<double>(ctx, opt, W_init, W.p);
+    if (rc) return rc;
+  }
+  if (hipMalloc(&hii.p, 8 * N) != hipSuccess)
+    return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
+  {
+    const double th = now_s();
+    rc = launch_hdiag(ctx, hii.p, nullptr);
+    if (rc) return rc;
     SBD_HIP_TRY(ctx, hipDeviceSynchronize());
-    (void)hipFree(W32.p);
-    W32.p = nullptr;
+    R->t_hdiag += now_s() - th;
   }
   if (lanczos)
     rc = lanczos_core(ctx, hii.p, W.p, opt->max_it, opt->max_nb, opt->eps, opt->verbose, &E, H,
-                      R, t0);
+                      R, t_solve0);
   else
     rc = davidson_core<double>(ctx, hii.p, W.p, opt->max_it, opt->max_nb, opt->eps,
-                               opt->max_time, opt->verbose, &E, H, R, t0);
+                               opt->max_time, opt->verbose, &E, H, R, t_solve0);
   if (rc) return rc;
-  R->t_davidson = now_s() - t0;
+  R->t_davidson = now_s() - t_solve0;
   if (hist && hist_cap > 0) {
     const size_t n = std::min((size_t)4 * hist_cap, hv.size());
     std::memcpy(hist, hv.data(), sizeof(double) * n);
   }
-  // E = <W, H W> with a fresh sigma (sbdiag.h:488-526)
-  if (hipMalloc(&HW.p, 8 * N) != hipSuccess)
-    return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-  rc = launch_sigma<double>(ctx, hii.p, W.p, HW.p, 0, nullptr);
-  if (rc) return rc;
+  // E = <W, H W> with a fresh sigma (sbdiag.h:488-526); the sigma's fused
+  // fixed-order <T, W> sum is that inner product
   double e = 0.0;
-  const double* X[1] = {W.p};
-  rc = vec_multidot<double>(ctx, X, 1, HW.p, N, &e, nullptr);
-  if (rc) return rc;
+  {
+    DevBuf<double> HW;
+    if (hipMalloc(&HW.p, 8 * N) != hipSuccess)
+      return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
+    rc = launch_sigma<double>(ctx, hii.p, W.p, HW.p, 0, nullptr, &e);
+    if (rc) return rc;
+  }
   if (opt->verbose) {
     printf(" Energy = %.16g\n", e);
     fflush(stdout);

@@ -281,6 +281,13 @@ template <typename VT>
 int vec_scale(sbd_ctx* ctx, VT* x, double s, uint64_t n, hipStream_t st);
 template <typename VS, typename VD>
 int vec_convert(sbd_ctx* ctx, const VS* x, VD* y, uint64_t n, hipStream_t st);
+// x = the normalised seeded vector of BasisInitVector's init = 1 (Randomize:
+// elements uniform_pm1(elem_hash(seed, i)), compensated squared norm, scaled
+// by 1 / norm in FP64, then stored as VT)
+template <typename VT>
+int vec_random(sbd_ctx* ctx, VT* x, uint64_t n, uint64_t seed, hipStream_t st);
+template <typename VT>
+int vec_fill(sbd_ctx* ctx, VT* x, uint64_t n, double v, hipStream_t st);
 
 // Seconds on a steady clock; records msg as the context's last error and
 // returns code.

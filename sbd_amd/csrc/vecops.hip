@@ -12,6 +12,7 @@
 // fixed grid and the block partials are summed in a fixed order, so results
 // are deterministic run to run.
 #include <algorithm>
+#include <cmath>
 #include <initializer_list>
 
 #include "sbd_internal.hpp"
@@ -448,6 +449,38 @@ __global__ void k_scale(VT* __restrict__ x, double s, uint64_t n) {
     x[i] = (VT)((double)x[i] * s);
 }
 
+// Start vectors on the device (BasisInitVector, davidson.h:144-184).
+// Randomize (dm_vector.h:101-191): element i is uniform_pm1(elem_hash(seed, i)),
+// computed where it is needed, so the squared norm is summed (compensated, per
+// thread, fixed-order) without storing the vector, and the vector is then
+// written already scaled by 1 / norm.
+__global__ __launch_bounds__(kVecBlock) void k_random_norm2(uint64_t n, uint64_t seed,
+                                                            double* __restrict__ part) {
+  Kahan k;
+  const Grid g = launch_grid();
+  for (uint64_t i = g.bid * (uint64_t)kVecBlock + threadIdx.x; i < n;
+       i += (uint64_t)g.nblk * kVecBlock) {
+    const double w = uniform_pm1(elem_hash(seed, i));
+    k.add(w * w);
+  }
+  double v[1] = {k.s};
+  block_partials<1>(g, v, 1, part);
+}
+
+template <typename VT>
+__global__ void k_random_fill(VT* __restrict__ x, uint64_t n, uint64_t seed, double factor) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = (VT)(uniform_pm1(elem_hash(seed, i)) * factor);
+}
+
+template <typename VT>
+__global__ void k_fill(VT* __restrict__ x, uint64_t n, VT v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
 template <typename VS, typename VD>
 __global__ void k_convert(const VS* __restrict__ x, VD* __restrict__ y, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -595,6 +628,29 @@ int vec_lincomb(sbd_ctx* ctx, const VT* const* C, int k, const double* u, double
 template <typename VT>
 int vec_scale(sbd_ctx* ctx, VT* x, double s, uint64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_scale<VT>, dim3(grid_for(n) * 2), dim3(kVecBlock), 0, st, x, s, n);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  return SBD_OK;
+}
+
+template <typename VT>
+int vec_random(sbd_ctx* ctx, VT* x, uint64_t n, uint64_t seed, hipStream_t st) {
+  int rc = ensure_scratch(ctx);
+  if (rc) return rc;
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_random_norm2, dim3(g), dim3(kVecBlock), 0, st, n, seed, ctx->d_partials);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  double n2 = 0.0;
+  rc = finish(ctx, g, 1, &n2, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_random_fill<VT>, dim3(grid_for(n) * 2), dim3(kVecBlock), 0, st, x, n, seed,
+                     1.0 / std::sqrt(n2));
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  return SBD_OK;
+}
+
+template <typename VT>
+int vec_fill(sbd_ctx* ctx, VT* x, uint64_t n, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill<VT>, dim3(grid_for(n) * 2), dim3(kVecBlock), 0, st, x, n, (VT)v);
   SBD_HIP_TRY(ctx, hipGetLastError());
   return SBD_OK;
 }
@@ -789,7 +845,9 @@ template int vec_gs_batch<double>(BatchScratch&, const GsArgs<double>*, int, dou
                                  uint64_t, double*, hipStream_t);                             \
   template int vec_lincomb<VT>(sbd_ctx*, const VT* const*, int, const double*, double, VT*,    \
                                uint64_t, double*, hipStream_t);                               \
-  template int vec_scale<VT>(sbd_ctx*, VT*, double, uint64_t, hipStream_t);
+  template int vec_scale<VT>(sbd_ctx*, VT*, double, uint64_t, hipStream_t);                    \
+  template int vec_random<VT>(sbd_ctx*, VT*, uint64_t, uint64_t, hipStream_t);                 \
+  template int vec_fill<VT>(sbd_ctx*, VT*, uint64_t, double, hipStream_t);
 SBD_INST(double)
 SBD_INST(float)
 template int vec_convert<float, double>(sbd_ctx*, const float*, double*, uint64_t, hipStream_t);

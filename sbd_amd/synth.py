@@ -197,6 +197,11 @@ WORKLOADS = {
     # workloads above most determinants are nearly isolated at small scale)
     "sci16": dict(norb=16, na=4, nb=4, ca=400, cb=400, rank=32, seed=6006),
     "sci16open": dict(norb=16, na=5, nb=3, ca=400, cb=300, rank=32, seed=7007),
+    # the sci58 integrals and sampled strings (1 % of them) plus every single
+    # excitation of the Hartree-Fock string: in the sampled sci58 basis the HF
+    # determinant has no coupled partner, so this is the connected norb-58 basis
+    # used for precision parity
+    "sci58hf": dict(norb=58, na=10, nb=10, ca=600, cb=600, rank=116, seed=3003, hf_singles=True),
 }
 
 
@@ -212,7 +217,17 @@ def make_workload(name: str, scale: float = 1.0):
     cb = max(2, int(round(w["cb"] * scale)))
     a = random_strings(L, w["na"], ca, w["seed"] + 1)
     b = random_strings(L, w["nb"], cb, w["seed"] + 2)
+    if w.get("hf_singles"):
+        a, b = (np.array(sorted(set(x.tolist()) | set(hf_singles(L, n).tolist())), dtype=np.uint64)
+                for x, n in ((a, w["na"]), (b, w["nb"])))
     return dict(norb=L, I0=I0, h=h, eri=eri, a=a, b=b)
+
+
+def hf_singles(L: int, n: int) -> np.ndarray:
+    """The single excitations of the Hartree-Fock string (orbitals 0..n-1)."""
+    hf = (1 << n) - 1
+    return np.array(sorted(hf ^ (1 << i) ^ (1 << a) for i in range(n) for a in range(n, L)),
+                    dtype=np.uint64)
 
 
 def write_fcidump(path: str, L: int, nelec: int, I0: float, h: np.ndarray, eri: np.ndarray,
