@@ -325,13 +325,21 @@ int sbd_diag_footprint(const sbd_ctx* ctx, const sbd_options* opt, uint64_t* byt
 /* ------------------------------------------------------------------ serving
  * A server runs many independent solves on the current GPU with continuous
  * batching: requests are admitted as device memory (mem_budget bytes) and
- * slots (max_active) allow, every admitted solve advances one Davidson step
- * per engine round, and each kind of device pass runs for all of them in one
- * launch.  A solve's result is bit-identical alone, in any mix, and to
- * sbd_diag with the same options.  The reference has no serving mode; the
- * solves are sbd::tpb::diag (chemistry/tpb/sbdiag.h:151-526, method 0, FP64).
- * Server calls are thread-safe. */
+ * slots (max_active) allow, and requests that arrive while others run join
+ * them as memory and slots free up.  Single-root FP64 solves (the
+ * reference's sbd::tpb::diag, chemistry/tpb/sbdiag.h:151-526, method 0) are
+ * advanced one Davidson step per engine round, each kind of device pass for
+ * all of them in one launch; each computes exactly as alone (same kernels,
+ * grids and fixed-order sums), so its energy is bit-identical alone, in any
+ * mix and to sbd_diag with the same options.  Requests that the batched form
+ * does not take -- precision 1 (FP32 Davidson + FP64 sweep) or nroots > 1
+ * (block Davidson, sbd_diag_roots) -- run whole on worker threads of their
+ * own, each on its own stream, concurrently with the batches; their results
+ * equal sbd_diag / sbd_diag_roots with the same options.  The reference has no
+ * serving mode.  Server calls are thread-safe. */
 typedef struct sbd_server sbd_server;
+
+#define SBD_MAX_ROOTS 13
 
 typedef struct {
   int norb;
@@ -342,16 +350,21 @@ typedef struct {
   size_t na;
   const uint64_t* b_strings;
   size_t nb;
-  sbd_options opt; /* method 0, precision 0; max_it, max_nb, eps, init, seed */
+  sbd_options opt; /* method 0; precision 0 or 1; max_it, max_nb, eps, init, seed,
+                      fp32_eps */
+  int nroots;      /* 0 or 1: lowest root; 2..SBD_MAX_ROOTS: block Davidson */
 } sbd_request;
 
 typedef struct {
-  double energy;   /* <W, H W> of the final vector, as sbd_diag */
-  double residual; /* |H W - E0 W| of the last step */
+  double energy;   /* <W, H W> of the final vector, as sbd_diag (lowest root) */
+  double residual; /* |H W - E0 W| of the last step (largest over the roots) */
   int converged;
   int steps;
   int status;      /* sbd_status of the solve */
   double t_submit, t_admit, t_done; /* seconds on the server clock */
+  int nroots;      /* roots solved for */
+  double energies[SBD_MAX_ROOTS]; /* <W_r, H W_r> per root, ascending */
+  int steps_fp32;  /* precision 1: steps taken in FP32 */
 } sbd_result;
 
 sbd_server* sbd_server_create(uint64_t mem_budget, int max_active, int* status);

@@ -66,20 +66,26 @@ class Report(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+MAX_ROOTS = 13
+
+
 class Request(C.Structure):
     _fields_ = [("norb", C.c_int), ("I0", C.c_double), ("h1", C.POINTER(C.c_double)),
                 ("eri_packed", C.POINTER(C.c_double)), ("a_strings", C.POINTER(C.c_uint64)),
                 ("na", C.c_size_t), ("b_strings", C.POINTER(C.c_uint64)), ("nb", C.c_size_t),
-                ("opt", Options)]
+                ("opt", Options), ("nroots", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("energy", C.c_double), ("residual", C.c_double), ("converged", C.c_int),
                 ("steps", C.c_int), ("status", C.c_int), ("t_submit", C.c_double),
-                ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("nroots", C.c_int),
+                ("energies", C.c_double * MAX_ROOTS), ("steps_fp32", C.c_int)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["energies"] = list(self.energies)[:max(1, self.nroots)]
+        return d
 
 
 def lib():
@@ -624,9 +630,11 @@ class Server:
 
     ``submit`` copies a request's integrals and strings and returns its id at
     once; ``wait`` blocks until that solve is done and returns its result
-    (energy, residual, converged, steps, status, and the submit / admit / done
-    times on the server clock).  A solve's result is bit-identical alone, in
-    any mix, and to ``MultTPB.diag`` with the same options.  ``policy`` is the
+    (energy, per-root energies, residual, converged, steps, status, and the
+    submit / admit / done times on the server clock).  A single-root FP64
+    solve's result is bit-identical alone, in any mix, and to ``MultTPB.diag``
+    with the same options; FP32 and multi-root requests run whole beside the
+    batches and equal ``MultTPB.diag`` / ``MultTPB.diag_roots``.  ``policy`` is the
     admission order: "fifo" (arrival order) or "sjf" (smallest footprint
     first; a request waiting ``age_limit`` seconds goes first)."""
 
@@ -642,14 +650,18 @@ class Server:
         _check(lib().sbd_server_set_policy(self._h, 0 if policy == "fifo" else 1,
                                            float(age_limit)))
 
-    def submit(self, norb, a, b, *, I0=0.0, h=None, eri=None, sbd: "SBD | None" = None) -> int:
+    def submit(self, norb, a, b, *, I0=0.0, h=None, eri=None, sbd: "SBD | None" = None,
+               nroots: int = 1) -> int:
+        """Queue a solve: the lowest root (nroots 1) or the nroots lowest
+        (block Davidson); sbd.precision 1 runs the FP32 Davidson + FP64 sweep."""
         sbd = sbd or SBD()
         h = np.ascontiguousarray(h, dtype=np.float64)
         eri = np.ascontiguousarray(eri, dtype=np.float64)
         a = np.ascontiguousarray(a, dtype=np.uint64)
         b = np.ascontiguousarray(b, dtype=np.uint64)
         rq = Request(int(norb), float(I0), _ptr(h, C.c_double), _ptr(eri, C.c_double),
-                     _ptr(a, C.c_uint64), len(a), _ptr(b, C.c_uint64), len(b), sbd.options())
+                     _ptr(a, C.c_uint64), len(a), _ptr(b, C.c_uint64), len(b), sbd.options(),
+                     int(nroots))
         rid = C.c_uint64(0)
         _check(lib().sbd_server_submit(self._h, C.byref(rq), C.byref(rid)))
         return rid.value

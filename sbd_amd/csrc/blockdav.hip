@@ -111,6 +111,7 @@ namespace sbdx {
 template <typename VT>
 int lowest_indices(sbd_ctx* ctx, const VT* d, uint64_t n, int nroots, std::vector<uint64_t>& out) {
   const int grid = 1024;
+  hipStream_t st = ctx->stream;
   double* bval = nullptr;
   uint64_t *bidx = nullptr, *excl = nullptr;
   SBD_HIP_TRY(ctx, hipMalloc(&bval, sizeof(double) * grid));
@@ -121,17 +122,18 @@ int lowest_indices(sbd_ctx* ctx, const VT* d, uint64_t n, int nroots, std::vecto
   std::vector<uint64_t> hi(grid);
   int rc = SBD_OK;
   for (int r = 0; r < nroots && !rc; r++) {
-    if (!out.empty() &&
-        hipMemcpy(excl, out.data(), sizeof(uint64_t) * out.size(), hipMemcpyHostToDevice) !=
-            hipSuccess) {
+    if (!out.empty() && hipMemcpyAsync(excl, out.data(), sizeof(uint64_t) * out.size(),
+                                       hipMemcpyHostToDevice, st) != hipSuccess) {
       rc = SBD_ERR_DEVICE;
       break;
     }
-    hipLaunchKernelGGL(k_argmin<VT>, dim3(grid), dim3(256), 0, nullptr, d, n, excl,
+    hipLaunchKernelGGL(k_argmin<VT>, dim3(grid), dim3(256), 0, st, d, n, excl,
                        (int)out.size(), bval, bidx);
-    if (hipMemcpy(hv.data(), bval, sizeof(double) * grid, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(hi.data(), bidx, sizeof(uint64_t) * grid, hipMemcpyDeviceToHost) !=
-            hipSuccess) {
+    if (hipMemcpyAsync(hv.data(), bval, sizeof(double) * grid, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(hi.data(), bidx, sizeof(uint64_t) * grid, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
       rc = SBD_ERR_DEVICE;
       break;
     }
@@ -156,29 +158,30 @@ int lowest_indices(sbd_ctx* ctx, const VT* d, uint64_t n, int nroots, std::vecto
 template <typename VT>
 static int orthonormalise(sbd_ctx* ctx, const std::vector<VT*>& C, int m, VT* t, uint64_t N,
                           bool* ok) {
+  hipStream_t st = ctx->stream;
   *ok = false;
   std::vector<double> o(kMaxVecs + 1, 0.0);
   double n2 = 0.0;
   const VT* tp[1] = {t};
-  int rc = vec_multidot<VT>(ctx, tp, 1, t, N, &n2, nullptr);
+  int rc = vec_multidot<VT>(ctx, tp, 1, t, N, &n2, st);
   if (rc) return rc;
   if (!(n2 > 0.0)) return SBD_OK;
   for (int pass = 0; pass < 2; pass++) {
     if (m > 0) {
       const auto Cc = cptr(C, m);
-      rc = vec_multidot<VT>(ctx, Cc.data(), m, t, N, o.data(), nullptr);
+      rc = vec_multidot<VT>(ctx, Cc.data(), m, t, N, o.data(), st);
       if (rc) return rc;
       double p2 = n2;
       for (int j = 0; j < m; j++) p2 -= o[j] * o[j];
       if (!(p2 > 1e-20 * n2)) return SBD_OK;  // nothing left outside the span
       double nn = 0.0;
-      rc = vec_gs_update<VT>(ctx, Cc.data(), m, o.data(), t, 1.0, N, &nn, nullptr);
+      rc = vec_gs_update<VT>(ctx, Cc.data(), m, o.data(), t, 1.0, N, &nn, st);
       if (rc) return rc;
       n2 = nn;
     }
   }
   if (!(n2 > 0.0)) return SBD_OK;
-  rc = vec_scale<VT>(ctx, t, 1.0 / std::sqrt(n2), N, nullptr);
+  rc = vec_scale<VT>(ctx, t, 1.0 / std::sqrt(n2), N, st);
   if (rc) return rc;
   *ok = true;
   return SBD_OK;
@@ -191,6 +194,7 @@ int block_davidson(sbd_ctx* ctx, const VT* dii, int nroots, std::vector<VT*>& X,
                    std::vector<double>& theta_out, std::vector<double>& res_out, int* steps_out,
                    std::vector<double>* hist) {
   const uint64_t N = ctx->N;
+  hipStream_t st = ctx->stream;
   const int R = nroots;
   const int M = std::min(std::max(max_sub, 3 * R), kMaxVecs);
   std::vector<Buf<VT>> Cb(M), HCb(M), Yb(4 * R);
@@ -209,7 +213,7 @@ int block_davidson(sbd_ctx* ctx, const VT* dii, int nroots, std::vector<VT*>& X,
   // start subspace: the given vectors, orthonormalised
   int m = 0;
   for (int r = 0; r < R; r++) {
-    SBD_HIP_TRY(ctx, hipMemcpy(C[m], X[r], sizeof(VT) * N, hipMemcpyDeviceToDevice));
+    SBD_HIP_TRY(ctx, hipMemcpyAsync(C[m], X[r], sizeof(VT) * N, hipMemcpyDeviceToDevice, st));
     bool ok = false;
     int rc = orthonormalise<VT>(ctx, C, m, C[m], N, &ok);
     if (rc) return rc;
@@ -219,12 +223,12 @@ int block_davidson(sbd_ctx* ctx, const VT* dii, int nroots, std::vector<VT*>& X,
   // sigma of the start vectors and their projected matrix
   auto add_columns = [&](int j0, int j1) -> int {
     for (int j = j0; j < j1; j++) {
-      int rc = launch_sigma<VT>(ctx, dii, C[j], HC[j], 0, nullptr);
+      int rc = launch_sigma<VT>(ctx, dii, C[j], HC[j], 0, st);
       if (rc) return rc;
     }
     for (int j = j0; j < j1; j++) {
       const auto Cc = cptr(C, j1);
-      int rc = vec_multidot<VT>(ctx, Cc.data(), j1, HC[j], N, o.data(), nullptr);
+      int rc = vec_multidot<VT>(ctx, Cc.data(), j1, HC[j], N, o.data(), st);
       if (rc) return rc;
       for (int i = 0; i < j1; i++) {
         H[i + (size_t)M * j] = o[i];
@@ -279,10 +283,10 @@ int block_davidson(sbd_ctx* ctx, const VT* dii, int nroots, std::vector<VT*>& X,
       const auto Cc = cptr(C, m), HCc = cptr(HC, m);
       for (int k = 0; k < K; k++) {
         double n2 = 0.0;
-        rc = vec_lincomb<VT>(ctx, Cc.data(), m, &Q[(size_t)k * m], 1.0, Yb[k].p, N, &n2, nullptr);
+        rc = vec_lincomb<VT>(ctx, Cc.data(), m, &Q[(size_t)k * m], 1.0, Yb[k].p, N, &n2, st);
         if (!rc)
           rc = vec_lincomb<VT>(ctx, HCc.data(), m, &Q[(size_t)k * m], 1.0, Yb[2 * R + k].p, N,
-                               &n2, nullptr);
+                               &n2, st);
         if (rc) return rc;
       }
       for (int k = 0; k < K; k++) {
@@ -294,7 +298,7 @@ int block_davidson(sbd_ctx* ctx, const VT* dii, int nroots, std::vector<VT*>& X,
       m = K;
       for (int j = 0; j < m; j++) {
         const auto Cm = cptr(C, m);
-        rc = vec_multidot<VT>(ctx, Cm.data(), m, HC[j], N, o.data(), nullptr);
+        rc = vec_multidot<VT>(ctx, Cm.data(), m, HC[j], N, o.data(), st);
         if (rc) return rc;
         for (int i = 0; i < m; i++) {
           H[i + (size_t)M * j] = o[i];
@@ -317,7 +321,7 @@ int block_davidson(sbd_ctx* ctx, const VT* dii, int nroots, std::vector<VT*>& X,
       VT* t = (m + added < M) ? C[m + added] : Yb[0].p;
       const auto Cc = cptr(C, m), HCc = cptr(HC, m);
       rc = vec_ritz_precond<VT>(ctx, Cc.data(), HCc.data(), m, u.data(), th[r], dii, t, N,
-                                o.data(), nullptr);
+                                o.data(), st);
       if (rc) return rc;
       res[r] = std::sqrt(o[0]);
       if (res[r] < eps) continue;
@@ -361,11 +365,11 @@ int block_davidson(sbd_ctx* ctx, const VT* dii, int nroots, std::vector<VT*>& X,
     for (int i = 0; i < m; i++) u[i] = U[i + (size_t)M * r];
     const auto Cc = cptr(C, m);
     double n2 = 0.0;
-    rc = vec_lincomb<VT>(ctx, Cc.data(), m, u.data(), 1.0, X[r], N, &n2, nullptr);
-    if (!rc) rc = vec_scale<VT>(ctx, X[r], 1.0 / std::sqrt(n2), N, nullptr);
+    rc = vec_lincomb<VT>(ctx, Cc.data(), m, u.data(), 1.0, X[r], N, &n2, st);
+    if (!rc) rc = vec_scale<VT>(ctx, X[r], 1.0 / std::sqrt(n2), N, st);
     if (rc) return rc;
   }
-  if (hipDeviceSynchronize() != hipSuccess) return SBD_ERR_DEVICE;
+  if (hipStreamSynchronize(st) != hipSuccess) return SBD_ERR_DEVICE;
   theta_out.assign(th.begin(), th.begin() + R);
   res_out = res;
   *steps_out = steps;
@@ -389,6 +393,7 @@ int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* ene
   if (rc) return rc;
   R->t_tables = clock_s() - t0;
   const uint64_t N = ctx->N;
+  hipStream_t st = ctx->stream;
   if ((uint64_t)nroots > N) return SBD_ERR_ARG;
   Buf<double> hii;
   std::vector<Buf<double>> Xb(nroots);
@@ -396,9 +401,9 @@ int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* ene
   for (auto& x : Xb)
     if (hipMalloc(&x.p, 8 * N) != hipSuccess) return SBD_ERR_MEMORY;
   const double t1 = clock_s();
-  rc = launch_hdiag(ctx, hii.p, nullptr);
+  rc = launch_hdiag(ctx, hii.p, st);
   if (rc) return rc;
-  if (hipDeviceSynchronize() != hipSuccess) return SBD_ERR_DEVICE;
+  if (hipStreamSynchronize(st) != hipSuccess) return SBD_ERR_DEVICE;
   R->t_hdiag = clock_s() - t1;
   // start vectors: unit vectors on the lowest diagonal elements (init 0) or
   // the seeded random stream with seed, seed + 1, ... (init 1)
@@ -408,14 +413,15 @@ int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* ene
     rc = lowest_indices<double>(ctx, hii.p, N, nroots, idx);
     if (rc) return rc;
     for (int r = 0; r < nroots; r++) {
-      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, nullptr, Xb[r].p, N, 0.0);
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, st, Xb[r].p, N, 0.0);
       const double one = 1.0;
-      if (hipMemcpy(Xb[r].p + idx[r], &one, 8, hipMemcpyHostToDevice) != hipSuccess)
+      if (hipMemcpyAsync(Xb[r].p + idx[r], &one, 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
         return SBD_ERR_DEVICE;
     }
   } else {
     for (int r = 0; r < nroots; r++)
-      hipLaunchKernelGGL(k_random_vec<double>, dim3(g), dim3(256), 0, nullptr, Xb[r].p, N,
+      hipLaunchKernelGGL(k_random_vec<double>, dim3(g), dim3(256), 0, st, Xb[r].p, N,
                          opt->seed + (uint64_t)r);
   }
   std::vector<double> hv, th, res;
@@ -430,9 +436,9 @@ int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* ene
     if (hipMalloc(&hii32.p, 4 * N) != hipSuccess) return SBD_ERR_MEMORY;
     for (auto& x : X32)
       if (hipMalloc(&x.p, 4 * N) != hipSuccess) return SBD_ERR_MEMORY;
-    rc = vec_convert<double, float>(ctx, hii.p, hii32.p, N, nullptr);
+    rc = vec_convert<double, float>(ctx, hii.p, hii32.p, N, st);
     for (int r = 0; r < nroots && !rc; r++)
-      rc = vec_convert<double, float>(ctx, Xb[r].p, X32[r].p, N, nullptr);
+      rc = vec_convert<double, float>(ctx, Xb[r].p, X32[r].p, N, st);
     if (rc) return rc;
     std::vector<float*> Xp(nroots);
     for (int r = 0; r < nroots; r++) Xp[r] = X32[r].p;
@@ -442,7 +448,7 @@ int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* ene
     if (rc && rc != SBD_ERR_NOT_CONVERGED) return rc;
     R->steps_fp32 = steps;
     for (int r = 0; r < nroots; r++) {
-      rc = vec_convert<float, double>(ctx, X32[r].p, Xb[r].p, N, nullptr);
+      rc = vec_convert<float, double>(ctx, X32[r].p, Xb[r].p, N, st);
       if (rc) return rc;
     }
   }
@@ -458,13 +464,14 @@ int sbd_diag_roots(sbd_ctx* ctx, const sbd_options* opt, int nroots, double* ene
   Buf<double> HX;
   if (hipMalloc(&HX.p, 8 * N) != hipSuccess) return SBD_ERR_MEMORY;
   for (int r = 0; r < nroots; r++) {
-    rc = launch_sigma<double>(ctx, hii.p, Xp[r], HX.p, 0, nullptr);
+    rc = launch_sigma<double>(ctx, hii.p, Xp[r], HX.p, 0, st);
     const double* Xc[1] = {Xp[r]};
-    if (!rc) rc = vec_multidot<double>(ctx, Xc, 1, HX.p, N, &energies[r], nullptr);
+    if (!rc) rc = vec_multidot<double>(ctx, Xc, 1, HX.p, N, &energies[r], st);
     if (rc) return rc;
     if (residuals) residuals[r] = res[r];
-    if (W_out && hipMemcpy(W_out + (size_t)r * N, Xp[r], 8 * N, hipMemcpyDeviceToHost) !=
-                     hipSuccess)
+    if (W_out && (hipMemcpyAsync(W_out + (size_t)r * N, Xp[r], 8 * N, hipMemcpyDeviceToHost,
+                                 st) != hipSuccess ||
+                  hipStreamSynchronize(st) != hipSuccess))
       return SBD_ERR_DEVICE;
   }
   if (hist && hist_cap > 0) {

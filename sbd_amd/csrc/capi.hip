@@ -221,7 +221,7 @@ sbd_ctx* create_common(int norb, double I0, std::vector<double>&& I1, std::vecto
   if (bad(hipMemcpy(ctx->d_K, K.data(), 8 * K.size(), hipMemcpyHostToDevice))) return nullptr;
   int rc = build_gtable(ctx, nullptr);
   if (!rc) rc = sigma_set_lds_limit(ctx);
-  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = SBD_ERR_DEVICE;
+  if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) rc = SBD_ERR_DEVICE;
   if (rc) {
     if (status) *status = rc;
     sbd_ctx_destroy(ctx);
@@ -261,7 +261,7 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
                   double max_time, int verbose, double* energy, std::vector<double>* hist,
                   sbd_report* rep, double t_start) {
   const uint64_t N = ctx->N;
-  hipStream_t st = nullptr;
+  hipStream_t st = ctx->stream;
   if (nb < 1 || nb > kMaxVecs) return set_err(ctx, SBD_ERR_ARG, "max_nb must be in [1, 40]");
   if (max_it < 1) return set_err(ctx, SBD_ERR_ARG, "max_it must be >= 1");
   nb = fit_subspace(nb, sizeof(VT) * N);
@@ -421,7 +421,7 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
 
 }  // namespace
 
-namespace {
+namespace sbdx {
 // W[0, N) = the start vector (BasisInitVector, davidson.h:144-184) in the
 // element type of W, built on the device: the caller's (host, FP64; a loaded
 // wavefunction, sbdiag.h:339-347), else e_0 (init 0) or the seeded Randomize
@@ -429,10 +429,11 @@ namespace {
 template <typename VT>
 int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT* W) {
   const uint64_t N = ctx->N;
+  hipStream_t st = ctx->stream;
   int rc = SBD_OK;
   if (W_init) {
     if (sizeof(VT) == 8) {
-      SBD_HIP_TRY(ctx, hipMemcpy(W, W_init, 8 * N, hipMemcpyHostToDevice));
+      SBD_HIP_TRY(ctx, hipMemcpyAsync(W, W_init, 8 * N, hipMemcpyHostToDevice, st));
     } else {
       // through an FP64 staging chunk, converted in place
       const uint64_t chunk = std::min<uint64_t>(N, 1ULL << 25);
@@ -441,15 +442,15 @@ int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT*
         return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
       for (uint64_t i0 = 0; i0 < N && !rc; i0 += chunk) {
         const uint64_t n = std::min(chunk, N - i0);
-        SBD_HIP_TRY(ctx, hipMemcpy(stage.p, W_init + i0, 8 * n, hipMemcpyHostToDevice));
-        rc = vec_convert<double, VT>(ctx, stage.p, W + i0, n, nullptr);
-        if (!rc && hipDeviceSynchronize() != hipSuccess) rc = SBD_ERR_DEVICE;
+        SBD_HIP_TRY(ctx, hipMemcpyAsync(stage.p, W_init + i0, 8 * n, hipMemcpyHostToDevice, st));
+        rc = vec_convert<double, VT>(ctx, stage.p, W + i0, n, st);
+        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = SBD_ERR_DEVICE;
       }
     }
   } else if (opt->init == 0) {
-    rc = vec_fill<VT>(ctx, W, N, 0.0, nullptr);
+    rc = vec_fill<VT>(ctx, W, N, 0.0, st);
     const VT one = (VT)1;
-    if (!rc) SBD_HIP_TRY(ctx, hipMemcpy(W, &one, sizeof(VT), hipMemcpyHostToDevice));
+    if (!rc) SBD_HIP_TRY(ctx, hipMemcpyAsync(W, &one, sizeof(VT), hipMemcpyHostToDevice, st));
   } else {
     uint64_t seed = opt->seed;
     if (seed == 0) {
@@ -458,12 +459,14 @@ int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT*
                               .count();
       seed = elem_hash(ns, 0);  // sbd_time_seed at rank 0
     }
-    rc = vec_random<VT>(ctx, W, N, seed, nullptr);
+    rc = vec_random<VT>(ctx, W, N, seed, st);
   }
-  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = SBD_ERR_DEVICE;
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = SBD_ERR_DEVICE;
   return rc;
 }
-}  // namespace
+template int start_vector<double>(sbd_ctx*, const sbd_options*, const double*, double*);
+template int start_vector<float>(sbd_ctx*, const sbd_options*, const double*, float*);
+}  // namespace sbdx
 
 extern "C" {
 
@@ -826,10 +829,10 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
   sbd_report* R = rep ? rep : &local;
   std::memset(R, 0, sizeof(sbd_report));
   const uint64_t N = ctx->N;
+  hipStream_t st = ctx->stream;
   double t0 = now_s();
   int rc = sbd_make_helpers(ctx);
   if (rc) return rc;
-  SBD_HIP_TRY(ctx, hipDeviceSynchronize());
   R->t_tables = now_s() - t0;
   // Device memory is held to what each phase needs: the FP32 phase holds its
   // vectors and FP32 diagonal only, and frees them (but W) before the FP64
@@ -850,9 +853,9 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
       rc = start_vector<float>(ctx, opt, W_init, W32.p);
       if (rc) return rc;
       const double th = now_s();
-      rc = launch_hdiag_f32(ctx, hii32.p, nullptr);
+      rc = launch_hdiag_f32(ctx, hii32.p, st);
       if (rc) return rc;
-      SBD_HIP_TRY(ctx, hipDeviceSynchronize());
+      SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
       R->t_hdiag += now_s() - th;
       const double e32 = std::max(opt->fp32_eps, opt->eps);
       rc = davidson_core<float>(ctx, hii32.p, W32.p, opt->max_it, opt->max_nb, e32, opt->max_time,
@@ -862,17 +865,17 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
     }
     if (hipMalloc(&W.p, 8 * N) != hipSuccess)
       return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-    rc = vec_convert<float, double>(ctx, W32.p, W.p, N, nullptr);
+    rc = vec_convert<float, double>(ctx, W32.p, W.p, N, st);
     if (rc) return rc;
-    SBD_HIP_TRY(ctx, hipDeviceSynchronize());
+    SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
     (void)hipFree(W32.p);
     W32.p = nullptr;
     // renormalise in FP64: the FP32 vector's norm is 1 only to FP32 accuracy,
     // and the Davidson start vector is taken as normalised
     const double* Wc[1] = {W.p};
     double n2 = 0.0;
-    rc = vec_multidot<double>(ctx, Wc, 1, W.p, N, &n2, nullptr);
-    if (!rc && n2 > 0.0) rc = vec_scale<double>(ctx, W.p, 1.0 / std::sqrt(n2), N, nullptr);
+    rc = vec_multidot<double>(ctx, Wc, 1, W.p, N, &n2, st);
+    if (!rc && n2 > 0.0) rc = vec_scale<double>(ctx, W.p, 1.0 / std::sqrt(n2), N, st);
     if (rc) return rc;
   } else {
     if (hipMalloc(&W.p, 8 * N) != hipSuccess)
@@ -884,9 +887,9 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
     return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
   {
     const double th = now_s();
-    rc = launch_hdiag(ctx, hii.p, nullptr);
+    rc = launch_hdiag(ctx, hii.p, st);
     if (rc) return rc;
-    SBD_HIP_TRY(ctx, hipDeviceSynchronize());
+    SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
     R->t_hdiag += now_s() - th;
   }
   if (lanczos)
@@ -908,7 +911,7 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
     DevBuf<double> HW;
     if (hipMalloc(&HW.p, 8 * N) != hipSuccess)
       return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-    rc = launch_sigma<double>(ctx, hii.p, W.p, HW.p, 0, nullptr, &e);
+    rc = launch_sigma<double>(ctx, hii.p, W.p, HW.p, 0, st, &e);
     if (rc) return rc;
   }
   if (opt->verbose) {
@@ -916,7 +919,10 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
     fflush(stdout);
   }
   if (energy) *energy = e;
-  if (W_out) SBD_HIP_TRY(ctx, hipMemcpy(W_out, W.p, 8 * N, hipMemcpyDeviceToHost));
+  if (W_out) {
+    SBD_HIP_TRY(ctx, hipMemcpyAsync(W_out, W.p, 8 * N, hipMemcpyDeviceToHost, st));
+    SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
+  }
   return SBD_OK;
 }
 

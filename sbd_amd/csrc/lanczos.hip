@@ -38,7 +38,7 @@ int lanczos_core(sbd_ctx* ctx, const double* hii, double* W, int max_it, int nb,
                  int verbose, double* energy, std::vector<double>* hist, sbd_report* rep,
                  double t_start) {
   const uint64_t N = ctx->N;
-  hipStream_t st = nullptr;
+  hipStream_t st = ctx->stream;
   if (nb < 1 || nb > kMaxVecs) return set_err(ctx, SBD_ERR_ARG, "max_nb must be in [1, 40]");
   if (max_it < 1) return set_err(ctx, SBD_ERR_ARG, "max_it must be >= 1");
   // nb Lanczos vectors and HC; the HC buffer of step ib becomes C[ib+1] and

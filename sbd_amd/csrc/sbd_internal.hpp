@@ -231,6 +231,9 @@ struct sbd_ctx {
   size_t partial_cap = 0;
   int lds_max = 65536;
   int cu_count = 256;
+  // stream of the solver passes (Davidson, Lanczos, block Davidson, diag):
+  // the null stream unless a caller such as the serving engine sets its own
+  hipStream_t stream = nullptr;
   int sigma_kernel = 0;  // 0 = by sparsity, 1 = row kernel, 2 = per-element, 3 = rows, alpha x beta inline
   int ab_pass = -1;      // dense alpha x beta pass: -1 not yet decided, 0 no, 1 yes
   size_t ab_lds = 0;     // its LDS bytes per workgroup
@@ -288,6 +291,12 @@ template <typename VT>
 int vec_random(sbd_ctx* ctx, VT* x, uint64_t n, uint64_t seed, hipStream_t st);
 template <typename VT>
 int vec_fill(sbd_ctx* ctx, VT* x, uint64_t n, double v, hipStream_t st);
+
+// W[0, N) = the start vector of sbd_diag in W's element type, on ctx->stream
+// (synchronised): the caller's W_init (host FP64) if given, else
+// BasisInitVector's init 0 (e_0) or init 1 (Randomize with opt->seed).
+template <typename VT>
+int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT* W);
 
 // Seconds on a steady clock; records msg as the context's last error and
 // returns code.

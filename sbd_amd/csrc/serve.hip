@@ -2,22 +2,32 @@
 // batching.
 //
 // Requests arrive at any time (sbd_server_submit).  An admission thread takes
-// them in arrival order, reserves their device footprint against the budget,
-// builds their context (tables, diagonal, start vector, Davidson buffers) and
-// hands each to one of a few engine lanes (a thread with its own stream),
-// balancing the lanes by estimated work per step so that light solves do not
-// wait each round for a heavy one.  A lane advances every solve it holds one
-// Davidson pass at a time (DavidsonJob, davidson_job.hpp) and runs each kind of
-// pass for all of them in one launch: one sigma launch (k_sigma_batch), one
-// Ritz launch, one Gram-Schmidt launch per round; the lanes' launches overlap
-// on the device.  A solve that finishes frees its memory at once, and requests
-// that arrived meanwhile are admitted as memory and slots allow, so their work
-// joins the next round.
+// them in arrival order (or smallest first), reserves their device footprint
+// against the budget and builds their context (tables on the device).
 //
-// Each solve takes the same number of workgroups in a batch as it would
-// alone and sums its own partials in a fixed order, so its iterates -- and its
-// energy -- are bit-identical alone, in any mix, and to sbd_diag with the same
-// options (FP64; init 0 or 1).
+//   * Single-root FP64 solves are batched: admission also makes their
+//     diagonal, start vector and Davidson buffers and hands each to one of a
+//     few engine lanes (a thread with its own stream), balancing the lanes by
+//     estimated work per step so that light solves do not wait each round for
+//     a heavy one.  A lane advances every solve it holds one Davidson pass at
+//     a time (DavidsonJob, davidson_job.hpp) and runs each kind of pass for
+//     all of them in one launch: one sigma launch (k_sigma_batch, or the
+//     solve's own launch where the batched kernel does not apply), one Ritz
+//     launch, one Gram-Schmidt launch per round; the lanes' launches overlap
+//     on the device.  Each solve takes the same kernels, grids and fixed-order
+//     sums in a batch as alone, so its iterates -- and its energy -- are
+//     bit-identical alone, in any mix, and to sbd_diag with the same options.
+//   * Requests the batched form does not take (precision 1: FP32 Davidson +
+//     FP64 sweep; nroots > 1: block Davidson) run whole on solo workers, each
+//     a thread with its own stream set as the context's stream, through
+//     sbd_diag / sbd_diag_roots: concurrent with the batches and with each
+//     other, and equal to those calls.
+//
+// Finished solves publish their result at once and hand their buffers and
+// context to a reaper thread: hipFree waits for the device, and doing it on
+// the lanes would stall every lane behind it.  The memory returns to the
+// budget when it has been freed, and queued requests are admitted as memory
+// and slots allow, so their work joins the next round.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +49,8 @@ using namespace sbdx;
 
 namespace {
 
-constexpr int kServerLanes = 8;  // engine lanes (threads + streams)
+constexpr int kServerLanes = 8;  // engine lanes (threads + streams) of batched solves
+constexpr int kSoloWorkers = 4;  // threads + streams of whole (FP32 / multi-root) solves
 
 double clock_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -52,11 +63,13 @@ struct Request {
   std::vector<double> h1, eri;
   std::vector<uint64_t> a, b;
   sbd_options opt{};
+  int nroots = 1;
   double t_submit = 0.0;
   uint64_t bytes = 0;  // reserved device footprint
+  bool batched() const { return opt.precision == 0 && nroots == 1; }
 };
 
-// One admitted solve with its device buffers.
+// One admitted solve with its device buffers (batched solves only).
 struct Solve {
   Request req;
   sbd_ctx* ctx = nullptr;
@@ -84,16 +97,26 @@ struct Solve {
   }
 };
 
-// Device bytes of one solve before its tables exist: 2 max_nb + 2 vectors
-// (C, HC, spare, hii), integrals, and tables and caches estimated for random
-// strings (each possible excitation partner present with probability
-// strings / C(norb, n)).  Corrected to the exact figure once the tables are
-// built.
+// Vectors a request's solve holds at its peak, in units of N FP64 elements:
+// batched and single-root solves 2 max_nb + 2 (C, HC, spare, hii); block
+// Davidson 2 M + 5 R + 2 (C, HC, restart buffers, the root vectors, hii and
+// H W), M = the subspace bound of sbd_diag_roots.
+double vectors_of(const Request& r) {
+  const double nb = (double)r.opt.max_nb;
+  if (r.nroots <= 1) return 2 * nb + 2;
+  const double R = (double)r.nroots;
+  const double M = std::min(std::max(nb, 3 * R), (double)kMaxVecs);
+  return 2 * M + 5 * R + 2;
+}
+
+// Device bytes of one solve before its tables exist: its vectors, integrals,
+// and tables and caches estimated for random strings (each possible
+// excitation partner present with probability strings / C(norb, n)).
+// Corrected to the exact figure once the tables are built.
 uint64_t footprint(const Request& r) {
   const uint64_t N = (uint64_t)r.a.size() * r.b.size();
-  const uint64_t nb = (uint64_t)r.opt.max_nb;
   const uint64_t L = (uint64_t)r.norb;
-  double bytes = (double)(2 * nb + 2) * N * 8;
+  double bytes = vectors_of(r) * N * 8;
   bytes += 8.0 * (4 * L * L + r.eri.size() + 2 * L * L + 4 * L * L * L + L * L * L * L);
   auto est = [&](const std::vector<uint64_t>& s, double& singles) {
     const double n = s.empty() ? 0 : (double)__builtin_popcountll(s[0]);
@@ -106,20 +129,21 @@ uint64_t footprint(const Request& r) {
   };
   double sa = 0, sb = 0;
   const double recs = est(r.a, sa) + est(r.b, sb);
-  bytes += 60.0 * recs + 16.0 * (r.a.size() + r.b.size() + 2);
+  bytes += 64.0 * recs + 16.0 * (r.a.size() + r.b.size() + 2);
   bytes += 8.0 * (sa * r.b.size() + r.a.size() * sb);
   return (uint64_t)bytes;
 }
 
 // The same after the tables are built, from the actual counts.
-uint64_t footprint_built(const sbd_ctx* c, int max_nb) {
+uint64_t footprint_built(const sbd_ctx* c, const Request& r) {
   const uint64_t N = c->N, L = (uint64_t)c->norb;
-  uint64_t bytes = (2 * (uint64_t)max_nb + 2) * N * 8;
+  uint64_t bytes = (uint64_t)(vectors_of(r) * (double)N * 8);
   bytes += 8 * (4 * L * L + c->eri.size() + 2 * L * L + 4 * L * L * L);
   if (c->d_Vab) bytes += 8 * L * L * L * L;
   bytes += 16 * (c->ta.n_s + c->ta.n_d + c->tb.n_s + c->tb.n_d);  // records
   bytes += 2 * 36 * (c->tb.n_d + c->nb);                          // ELL copies (bounded)
   bytes += 8 * 8 * (c->na + c->nb + 2);                           // offsets, orders, strings
+  bytes += 4 * (uint64_t)c->tb.max_s * c->nb;                     // dense-pass records
   if (c->d_Ea) bytes += 8 * c->ta.n_s * c->nb;
   if (c->d_Eb) bytes += 8 * c->na * c->tb.n_s;
   return bytes;
@@ -127,9 +151,8 @@ uint64_t footprint_built(const sbd_ctx* c, int max_nb) {
 
 }  // namespace
 
-// One engine lane: its own thread, stream and scratch, advancing the solves
-// assigned to it.  Solves are spread over the lanes by estimated step cost so
-// that light solves do not wait every round for a heavy one.
+// One engine lane: its own thread, stream and scratch, advancing the batched
+// solves assigned to it.
 struct Lane {
   std::deque<Solve*> ready;
   hipStream_t st = nullptr;
@@ -144,20 +167,31 @@ struct Lane {
   }
 };
 
+// One solo worker: a thread and a stream running whole solves.
+struct Solo {
+  hipStream_t st = nullptr;
+  std::thread th;
+};
+
 struct sbd_server {
   int device = 0;
   uint64_t budget = 0, used = 0;
   int max_active = 0;
-  std::mutex mu;
-  std::condition_variable cv_admit, cv_engine, cv_done;
+  mutable std::mutex mu;
+  std::condition_variable cv_admit, cv_engine, cv_done, cv_solo, cv_reap;
   std::deque<Request> pending;
   std::vector<Lane> lanes;
-  int preparing = 0, running = 0, queued = 0;  // queued: prepared, not yet on a lane's batch
+  std::vector<Solo> solos;
+  std::deque<Solve*> solo_ready;  // prepared whole solves (context only)
+  std::deque<Solve*> reap;        // finished solves whose memory is to be freed
+  int preparing = 0, running = 0, queued = 0;  // queued: prepared, not yet running
+  int reaping = 0;
   std::map<uint64_t, sbd_result> results;
   std::map<uint64_t, int> state;  // 0 queued, 1 admitted, 2 done
   bool stopping = false;
   uint64_t next_id = 1;
-  std::thread admitter;
+  std::thread admitter, reaper;
+  hipStream_t st_admit = nullptr;
   double t0 = 0.0;
   uint64_t rounds = 0, peak_active = 0;
   std::string last_error;
@@ -179,21 +213,17 @@ struct sbd_server {
     return best;
   }
 
-  void finish(Solve* s, double energy) {
-    sbd_result r{};
-    r.energy = energy;
-    r.converged = s->job.converged ? 1 : 0;
-    r.steps = s->job.steps;
-    r.residual = s->job.normR;
+  // Publish the result now; the memory goes back to the budget once the
+  // reaper has freed it.
+  void finish(Solve* s, const sbd_result& res) {
+    sbd_result r = res;
     r.t_submit = s->req.t_submit;
     r.t_admit = s->t_admit;
     r.t_done = now();
     r.status = s->status;
-    s->release();
     std::lock_guard<std::mutex> g(mu);
-    used -= s->req.bytes;
-    lanes[s->lane].load -= s->cost;
-    {
+    if (s->req.batched()) {
+      lanes[s->lane].load -= s->cost;
       auto& cs = lanes[s->lane].costs;
       auto it = std::find(cs.begin(), cs.end(), s->cost);
       if (it != cs.end()) cs.erase(it);
@@ -201,12 +231,35 @@ struct sbd_server {
     results[s->req.id] = r;
     state[s->req.id] = 2;
     running--;
+    reaping++;
+    reap.push_back(s);
     cv_done.notify_all();
-    cv_admit.notify_all();
-    delete s;
+    cv_reap.notify_all();
   }
 
-  // ---- admission: context, tables, diagonal, start vector, buffers
+  void reap_loop() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      Solve* s = nullptr;
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv_reap.wait(g, [&] { return !reap.empty() || (stopping && reaping == 0); });
+        if (reap.empty()) return;
+        s = reap.front();
+        reap.pop_front();
+      }
+      s->release();
+      std::lock_guard<std::mutex> g(mu);
+      used -= s->req.bytes;
+      reaping--;
+      delete s;
+      cv_admit.notify_all();
+      cv_reap.notify_all();
+    }
+  }
+
+  // ---- admission: context and tables; for batched solves also the diagonal,
+  // start vector and Davidson buffers
   int prepare(Solve* s) {
     const Request& q = s->req;
     int st0 = 0;
@@ -215,6 +268,7 @@ struct sbd_server {
     if (!s->ctx) return st0 ? st0 : SBD_ERR_DEVICE;
     int rc = sbd_make_helpers(s->ctx);
     if (rc) return rc;
+    if (!q.batched()) return SBD_OK;
     const uint64_t N = s->ctx->N;
     const int nb = q.opt.max_nb;
     s->C.assign(nb, nullptr);
@@ -225,18 +279,12 @@ struct sbd_server {
     }
     if (hipMalloc(&s->spare, 8 * N) != hipSuccess || hipMalloc(&s->hii, 8 * N) != hipSuccess)
       return SBD_ERR_MEMORY;
-    rc = launch_hdiag(s->ctx, s->hii, nullptr);
+    // the diagonal and the start vector as sbd_diag makes them
+    // (BasisInitVector, davidson.h:144-184), on the admission stream
+    s->ctx->stream = st_admit;
+    rc = launch_hdiag(s->ctx, s->hii, st_admit);
+    if (!rc) rc = start_vector<double>(s->ctx, &q.opt, nullptr, s->C[0]);
     if (rc) return rc;
-    // start vector as sbd_diag makes it (BasisInitVector, davidson.h:144-184)
-    std::vector<double> W0(N, 0.0);
-    if (q.opt.init == 0) {
-      W0[0] = 1.0;
-    } else {
-      randomize(W0.data(), N, q.opt.seed);
-    }
-    if (hipMemcpy(s->C[0], W0.data(), 8 * N, hipMemcpyHostToDevice) != hipSuccess)
-      return SBD_ERR_DEVICE;
-    if (hipDeviceSynchronize() != hipSuccess) return SBD_ERR_DEVICE;
     s->job.init(s->C, s->HC, s->spare, s->hii, N, q.opt.max_it, q.opt.eps);
     const sbd_ctx* c = s->ctx;
     const double na = (double)c->na, nbv = (double)c->nb;
@@ -251,16 +299,19 @@ struct sbd_server {
       Request q;
       {
         std::unique_lock<std::mutex> g(mu);
+        size_t k = 0;
         cv_admit.wait(g, [&] {
           if (stopping && pending.empty()) return true;
           if (pending.empty()) return false;
-          const uint64_t need = pending[pick_pending()].bytes;
-          const bool slot = running + preparing + queued < max_active;
-          const bool mem = used + need <= budget || (running + preparing + queued == 0);
+          k = pick_pending();
+          const uint64_t need = pending[k].bytes;
+          const int active = running + preparing + queued;
+          const bool slot = active < max_active;
+          const bool mem = used + need <= budget || (active == 0 && reaping == 0);
           return slot && mem;
         });
         if (pending.empty()) return;  // stopping
-        const size_t k = pick_pending();
+        // admit exactly the request the predicate checked
         q = std::move(pending[k]);
         pending.erase(pending.begin() + (std::ptrdiff_t)k);
         used += q.bytes;
@@ -274,7 +325,7 @@ struct sbd_server {
       preparing--;
       if (!rc) {
         // replace the estimate by the actual footprint
-        const uint64_t actual = footprint_built(s->ctx, s->req.opt.max_nb);
+        const uint64_t actual = footprint_built(s->ctx, s->req);
         used = used - s->req.bytes + actual;
         s->req.bytes = actual;
         cv_admit.notify_all();
@@ -284,7 +335,13 @@ struct sbd_server {
         last_error = s->ctx ? s->ctx->last_error : "request preparation failed";
         running++;  // finish() counts it out
         g.unlock();
-        finish(s, 0.0);
+        finish(s, sbd_result{});
+        continue;
+      }
+      queued++;
+      if (!s->req.batched()) {
+        solo_ready.push_back(s);
+        cv_solo.notify_one();
         continue;
       }
       // A lane's round lasts about as long as its heaviest solve's step, so a
@@ -303,12 +360,53 @@ struct sbd_server {
       lanes[best].load += s->cost;
       lanes[best].costs.push_back(s->cost);
       lanes[best].ready.push_back(s);
-      queued++;
       cv_engine.notify_all();
     }
   }
 
-  // ---- engine: one round advances every active solve through its passes
+  // ---- solo workers: whole solves on their own stream
+  void solo_loop(int wi) {
+    (void)hipSetDevice(device);
+    Solo& w = solos[wi];
+    for (;;) {
+      Solve* s = nullptr;
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv_solo.wait(g, [&] {
+          return !solo_ready.empty() || (stopping && pending.empty() && preparing == 0);
+        });
+        if (solo_ready.empty()) return;
+        s = solo_ready.front();
+        solo_ready.pop_front();
+        s->t_admit = now();
+        queued--;
+        running++;
+        peak_active = std::max<uint64_t>(peak_active, (uint64_t)running);
+      }
+      s->ctx->stream = w.st;
+      sbd_result r{};
+      sbd_report rep{};
+      const Request& q = s->req;
+      int rc;
+      if (q.nroots > 1) {
+        double res[SBD_MAX_ROOTS];
+        rc = sbd_diag_roots(s->ctx, &q.opt, q.nroots, r.energies, res, nullptr, nullptr, 0, &rep);
+        r.energy = r.energies[0];
+      } else {
+        rc = sbd_diag(s->ctx, &q.opt, &r.energy, nullptr, nullptr, 0, &rep);
+        r.energies[0] = r.energy;
+      }
+      r.nroots = q.nroots;
+      r.residual = rep.residual;
+      r.converged = rep.converged;
+      r.steps = rep.steps;
+      r.steps_fp32 = rep.steps_fp32;
+      if (rc && rc != SBD_ERR_NOT_CONVERGED) s->status = rc;
+      finish(s, r);
+    }
+  }
+
+  // ---- engine: one round advances every active batched solve through its passes
   void engine_loop(int li) {
     (void)hipSetDevice(device);
     Lane& ln = lanes[li];
@@ -324,6 +422,7 @@ struct sbd_server {
           Solve* s = ln.ready.front();
           ln.ready.pop_front();
           s->t_admit = now();
+          s->ctx->stream = ln.st;
           queued--;
           running++;
           active.push_back(s);
@@ -453,9 +552,10 @@ struct sbd_server {
     }
   }
 
-  // final vector and E = <W, H W> with a fresh sigma, as sbd_diag does
+  // final vector and E = <W, H W> with a fresh sigma and its fused inner
+  // product, as sbd_diag does
   void complete(Solve* s, hipStream_t st) {
-    double E = 0.0;
+    sbd_result r{};
     if (!s->status) {
       const uint64_t N = s->ctx->N;
       auto& j = s->job;
@@ -465,15 +565,18 @@ struct sbd_server {
       int rc = vec_lincomb<double>(s->ctx, cp.data(), j.last_k, j.ritz_u(), 1.0, W, N, &n2, st);
       if (!rc) rc = vec_scale<double>(s->ctx, W, 1.0 / std::sqrt(n2), N, st);
       double* HW = j.HC[0];
-      if (!rc) rc = launch_sigma<double>(s->ctx, s->hii, W, HW, 0, st);
-      const double* X[1] = {W};
-      if (!rc) rc = vec_multidot<double>(s->ctx, X, 1, HW, N, &E, st);
+      if (!rc) rc = launch_sigma<double>(s->ctx, s->hii, W, HW, 0, st, &r.energy);
       if (rc) s->status = rc;
     }
+    r.nroots = 1;
+    r.energies[0] = r.energy;
+    r.converged = s->job.converged ? 1 : 0;
+    r.steps = s->job.steps;
+    r.residual = s->job.normR;
     // buffers may have been rotated by restarts
     s->C = s->job.C;
     s->spare = s->job.spare;
-    finish(s, E);
+    finish(s, r);
   }
 };
 
@@ -489,19 +592,29 @@ sbd_server* sbd_server_create(uint64_t mem_budget, int max_active, int* status) 
   sv->max_active = max_active;
   sv->t0 = clock_s();
   sv->lanes = std::vector<Lane>(std::min(kServerLanes, max_active));
+  sv->solos = std::vector<Solo>(std::min(kSoloWorkers, max_active));
   bool ok = hipGetDevice(&sv->device) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&sv->st_admit, hipStreamNonBlocking) == hipSuccess;
   for (Lane& l : sv->lanes)
     ok = ok && hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) == hipSuccess;
+  for (Solo& w : sv->solos)
+    ok = ok && hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking) == hipSuccess;
   if (!ok) {
     for (Lane& l : sv->lanes)
       if (l.st) (void)hipStreamDestroy(l.st);
+    for (Solo& w : sv->solos)
+      if (w.st) (void)hipStreamDestroy(w.st);
+    if (sv->st_admit) (void)hipStreamDestroy(sv->st_admit);
     if (status) *status = SBD_ERR_DEVICE;
     delete sv;
     return nullptr;
   }
   sv->admitter = std::thread([sv] { sv->admit_loop(); });
+  sv->reaper = std::thread([sv] { sv->reap_loop(); });
   for (int l = 0; l < (int)sv->lanes.size(); l++)
     sv->lanes[l].th = std::thread([sv, l] { sv->engine_loop(l); });
+  for (int w = 0; w < (int)sv->solos.size(); w++)
+    sv->solos[w].th = std::thread([sv, w] { sv->solo_loop(w); });
   if (status) *status = SBD_OK;
   return sv;
 }
@@ -510,8 +623,11 @@ int sbd_server_submit(sbd_server* sv, const sbd_request* rq, uint64_t* id) {
   if (!sv || !rq || !rq->h1 || !rq->eri_packed || !rq->a_strings || !rq->b_strings ||
       rq->norb < 1 || rq->norb > 64 || rq->na == 0 || rq->nb == 0)
     return SBD_ERR_ARG;
-  if (rq->opt.method != 0 || rq->opt.precision != 0 || rq->opt.max_nb < 1 ||
-      rq->opt.max_nb > kMaxVecs || rq->opt.max_it < 1 || (rq->opt.init != 0 && rq->opt.init != 1))
+  const int nroots = rq->nroots < 1 ? 1 : rq->nroots;
+  if (rq->opt.method != 0 || (rq->opt.precision != 0 && rq->opt.precision != 1) ||
+      rq->opt.max_nb < 1 || rq->opt.max_nb > kMaxVecs || rq->opt.max_it < 1 ||
+      (rq->opt.init != 0 && rq->opt.init != 1) || nroots > SBD_MAX_ROOTS ||
+      nroots > kMaxVecs / 3 || (uint64_t)nroots > (uint64_t)rq->na * rq->nb)
     return SBD_ERR_ARG;
   Request q;
   q.norb = rq->norb;
@@ -522,6 +638,7 @@ int sbd_server_submit(sbd_server* sv, const sbd_request* rq, uint64_t* id) {
   q.a.assign(rq->a_strings, rq->a_strings + rq->na);
   q.b.assign(rq->b_strings, rq->b_strings + rq->nb);
   q.opt = rq->opt;
+  q.nroots = nroots;
   q.bytes = footprint(q);
   std::lock_guard<std::mutex> g(sv->mu);
   if (sv->stopping) return SBD_ERR_STATE;
@@ -565,6 +682,7 @@ int sbd_server_set_policy(sbd_server* sv, int policy, double age_limit_s) {
 
 int sbd_server_stats(const sbd_server* sv, uint64_t* out) {
   if (!sv || !out) return SBD_ERR_ARG;
+  std::lock_guard<std::mutex> g(sv->mu);
   out[0] = sv->rounds;
   out[1] = sv->peak_active;
   return SBD_OK;
@@ -577,17 +695,30 @@ void sbd_server_destroy(sbd_server* sv) {
     sv->stopping = true;
     sv->cv_admit.notify_all();
     sv->cv_engine.notify_all();
+    sv->cv_solo.notify_all();
   }
   if (sv->admitter.joinable()) sv->admitter.join();
   {
     std::lock_guard<std::mutex> g(sv->mu);
     sv->cv_engine.notify_all();
+    sv->cv_solo.notify_all();
   }
-  for (Lane& l : sv->lanes) {
+  for (Lane& l : sv->lanes)
     if (l.th.joinable()) l.th.join();
+  for (Solo& w : sv->solos)
+    if (w.th.joinable()) w.th.join();
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->cv_reap.notify_all();
+  }
+  if (sv->reaper.joinable()) sv->reaper.join();
+  for (Lane& l : sv->lanes) {
     batch_free(l.sc);
     if (l.st) (void)hipStreamDestroy(l.st);
   }
+  for (Solo& w : sv->solos)
+    if (w.st) (void)hipStreamDestroy(w.st);
+  if (sv->st_admit) (void)hipStreamDestroy(sv->st_admit);
   delete sv;
 }
 

@@ -3,7 +3,8 @@ found by scipy's ARPACK Lanczos (scipy.sparse.linalg.eigsh) on a linear
 operator that applies the oracle's sigma: an energy independent of the
 Davidson restatement (the sigma itself is pinned element by element against
 second quantisation).  Run from the repository root after make_energies.py:
-    python tests/golden/make_eigsh.py"""
+    python tests/golden/make_eigsh.py [case ...]
+(no case names: every case that has no eigsh energy yet)."""
 import json
 import os
 import sys
@@ -18,7 +19,10 @@ from oracle import oracle as O  # noqa: E402
 
 path = os.path.join(ROOT, "tests", "golden", "energies.json")
 data = json.load(open(path))
+want = sys.argv[1:] or [k for k, d in data.items() if "energy_eigsh" not in d]
 for key, d in data.items():
+    if key not in want:
+        continue
     w = synth.make_workload(d["workload"], d["scale"])
     S = O.System.from_workload(w)
     hii = S.hdiag()
