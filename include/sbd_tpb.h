@@ -194,6 +194,17 @@ int sbd_sigma_f32(sbd_ctx* ctx, const float* hii, const float* Wk, float* Wb, in
 int sbd_inner(sbd_ctx* ctx, const double* x, const double* y, uint64_t n, double* out,
               void* stream);
 
+/* sbd_inner without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory at call time (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and grid as sbd_inner, so the same bits.  Its reduction
+ * scratch is a fixed device array of 1,088 doubles (8.5 KiB) per device,
+ * held until the process ends; any n fits (at most 1,024 block partials and
+ * the result).  Every call on a device shares that array: calls that overlap
+ * on one device race.  Synchronises the stream; *out is host memory. */
+int sbd_inner_noalloc(sbd_ctx* ctx, const double* x, const double* y, uint64_t n, double* out,
+                      void* stream);
+
 /* Single-root Davidson on device vectors (tpb/davidson.h:707-970): W is the
  * initial vector on entry and the normalised Ritz vector on exit; *energy is
  * the Ritz value of the last step.  hist (optional, host, 4*max_it*max_nb

@@ -32,7 +32,8 @@ EXPORTS = [
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
     "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_set_sigma_kernel", "sbd_make_helpers",
     "sbd_table_counts", "sbd_table_get",
-    "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_davidson",
+    "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_inner_noalloc",
+    "sbd_davidson",
     "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_correlation", "sbd_carryover",
     "sbd_diag_from",
     "sbd_save_wavefunction", "sbd_load_wavefunction", "sbd_singles_extend",
@@ -123,6 +124,7 @@ def lib():
     L.sbd_sigma.argtypes = [vp, vp, vp, vp, C.c_int, vp]
     L.sbd_sigma_f32.argtypes = [vp, vp, vp, vp, C.c_int, vp]
     L.sbd_inner.argtypes = [vp, vp, vp, C.c_uint64, dp, vp]
+    L.sbd_inner_noalloc.argtypes = [vp, vp, vp, C.c_uint64, dp, vp]
     L.sbd_davidson.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_lanczos.argtypes = [vp, vp, vp, C.POINTER(Options), dp, dp, C.POINTER(Report)]
     L.sbd_string_weights.argtypes = [vp, vp, dp, dp, vp]

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <initializer_list>
+#include <mutex>
 
 #include "sbd_internal.hpp"
 
@@ -854,3 +855,66 @@ template int vec_convert<float, double>(sbd_ctx*, const float*, double*, uint64_
 template int vec_convert<double, float>(sbd_ctx*, const double*, float*, uint64_t, hipStream_t);
 
 }  // namespace sbdx
+
+// ------------------------------------------- inner product without allocation
+//
+// sbd_inner_noalloc: the same kernels and grid as sbd_inner (k_multidot for
+// one vector, then k_finalize), with the block partials and the result slot
+// in a fixed device array instead of the context's lazily allocated buffers,
+// for callers that cannot allocate device memory.  Every partial the
+// finalize step reads is written by the same call first, so nothing needs
+// clearing between calls.  All calls on one device share the array: calls
+// that overlap on one device (other host threads, other streams) race.
+
+namespace {
+
+constexpr size_t kNoAllocDoubles = (size_t)sbdx::kVecGrid + 64;  // partials + result, padded
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoAllocDevices] = {};  // device address of scratch, per device
+
+// The current device's copy of scratch, looked up once per device with that
+// device current (the array's host-side name is not a device address).
+int noalloc_scratch(sbd_ctx* ctx, double** out) {
+  int dev = -1;
+  SBD_HIP_TRY(ctx, hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices)
+    return sbdx::set_err(ctx, SBD_ERR_DEVICE, "sbd_inner_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    SBD_HIP_TRY(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return sbdx::set_err(ctx, SBD_ERR_DEVICE, "sbd_inner_noalloc: no scratch address");
+    g_noalloc_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return SBD_OK;
+}
+
+}  // namespace
+
+extern "C" int sbd_inner_noalloc(sbd_ctx* ctx, const double* x, const double* y, uint64_t n,
+                                 double* out, void* stream) {
+  using namespace sbdx;
+  if (!ctx || !x || !y || !out) return SBD_ERR_ARG;
+  const hipStream_t st = (hipStream_t)stream;
+  const int g = grid_for(n);
+  if ((size_t)g + 1 > kNoAllocDoubles)
+    return set_err(ctx, SBD_ERR_ARG, "sbd_inner_noalloc: reduction exceeds the fixed scratch");
+  double* part = nullptr;
+  int rc = noalloc_scratch(ctx, &part);
+  if (rc) return rc;
+  double* res = part + g;
+  const double* X[1] = {x};
+  const bool packed = aligned16(X, 1) && aligned16<double>({y});
+  hipLaunchKernelGGL((k_multidot<double, 4>), dim3(g), dim3(kVecBlock), 0, st, pack(X, 1), 1, y,
+                     n, packed, part);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kVecBlock), 0, st, part, g, 1, res);
+  SBD_HIP_TRY(ctx, hipGetLastError());
+  SBD_HIP_TRY(ctx, hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, st));
+  SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
+  return SBD_OK;
+}
