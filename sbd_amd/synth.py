@@ -202,6 +202,10 @@ WORKLOADS = {
     # determinant has no coupled partner, so this is the connected norb-58 basis
     # used for precision parity
     "sci58hf": dict(norb=58, na=10, nb=10, ca=600, cb=600, rank=116, seed=3003, hf_singles=True),
+    # the same for the open-shell sci48open basis (1 % of its sampled strings
+    # plus the HF singles): the connected basis for multi-root parity
+    "sci48openhf": dict(norb=48, na=12, nb=10, ca=300, cb=250, rank=96, seed=5005,
+                        hf_singles=True),
 }
 
 

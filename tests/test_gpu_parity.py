@@ -224,6 +224,34 @@ def test_mixed_precision_energy(torch):
     m.close()
 
 
+def test_precision_solve_sci58(torch):
+    """The precision pattern (FP32 Davidson, then the FP64 sweep) on the sci58
+    integrals (norb 58, 10 + 10 electrons, rank 116): the sampled strings at
+    1 % plus every single excitation of the Hartree-Fock string, 1.17e6
+    determinants.  Both the mixed and the plain FP64 solve land within 1e-9 Ha
+    of the oracle's Davidson energy (tests/golden/energies.json "sci58hf",
+    itself within 5e-12 of ARPACK on the oracle's sigma), the mixed one after
+    FP32 steps."""
+    import json
+    import sbd_amd
+    gold = json.load(open(os.path.join(GOLDEN, "energies.json")))["sci58hf"]
+    say("precision solve sci58hf")
+    w = synth_case(gold["workload"], gold["scale"])
+    m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
+    assert m.N == gold["N"]
+    E64, _, r64 = m.diag(sbd_amd.SBD(max_it=100, max_nb=10, eps=1e-8, init=gold["init"]))
+    E32, _, r32 = m.diag(sbd_amd.SBD(max_it=100, max_nb=10, eps=1e-8, init=gold["init"],
+                                     precision=1, fp32_eps=1e-4))
+    say(f"sci58hf: E64 {E64:.12f} ({r64['steps']} steps), mixed {E32:.12f} "
+        f"({r32['steps_fp32']} FP32 + {r32['steps'] - r32['steps_fp32']} FP64 steps), "
+        f"oracle {gold['energy']:.12f}")
+    assert r64["converged"] == 1 and r32["converged"] == 1
+    assert r32["steps_fp32"] > 0
+    assert abs(E64 - gold["energy"]) < 1e-9, (E64, gold["energy"])
+    assert abs(E32 - gold["energy"]) < 1e-9, (E32, gold["energy"])
+    m.close()
+
+
 def test_large_sigma_properties(torch):
     """Full-size property checks on sci36 at 1/4 strings: symmetry <x,Hy> = <y,Hx>
     and linearity; both hold for any correct Hermitian sigma."""
