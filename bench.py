@@ -17,18 +17,33 @@ Printed JSON line:
                a bounded sample of bra rows of the same workload
   rdm          wall time of the full one- and two-particle RDMs (do_rdm != 0)
                of a normalised vector on the same workload
-  serving      Poisson streams of independent solves (norb 16-40, n 4-10,
+  dense        FCI norb 12 (245,025 dets, 1,424 terms/det): sigma GDet/s and
+               terms/s, Davidson to residual 1e-8 vs the oracle's energy, and
+               the oracle's own sigma and Davidson on the host; the
+               reference's C2H2 (100k TrimCI core, 6.19e7 dets) and H2O
+               (threshold 1e-6, 1.91e8 dets) inputs solved to residual 1e-6,
+               energies against the values the reference's READMEs hold
+  precision    norb 58, 10 + 10 electrons, the 60,000 x 60,000 sampled strings
+               plus the HF singles (3.66e9 dets): plain FP64 Davidson vs FP32
+               Davidson + FP64 sweep, energy error and time to 1e-6 Ha
+  serving      a Poisson stream of 256 independent solves (norb 16-40, n 4-10,
                200-8000 strings per spin) through the continuous-batching
                server vs the same stream solved one at a time: p50/p99
-               latency and solves/s, at overload (64 requests at 40/s; the
-               server admitting in arrival order and smallest-first) and at a
-               steady sub-saturation load (32 requests at 0.5/s)
+               latency and solves/s
+  multiroot    32 concurrent open-shell block-Davidson requests (4 roots,
+               norb 48, 12 + 10 electrons, the sci48open string counts x 0.1
+               with +-30 % jitter, lowest-diagonal start): FP32 vectors with FP64
+               Ritz / Gram-Schmidt vs the same requests in FP64, per-root
+               energy error, p50/p99 latency, solves/s
 """
 import argparse
+import gzip
 import json
 import math
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -88,6 +103,236 @@ def cpu_baseline(w, m, T_dev, Wb_dev, hii_dev, target_s):
             "sample": f"bra rows 0..{rows - 1} x {nb} beta strings ({dets:.3e} dets) of the "
                       f"same workload, one FP64 sigma, {dt:.1f}s, OpenMP, {model}",
             "sample_bitexact_vs_gpu": exact}
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def terms_per_sigma(m, counts):
+    """Off-diagonal matrix elements one sigma evaluates: each alpha pair with
+    every beta string, each beta pair with every alpha string, and each
+    (alpha single, beta single) pair."""
+    ca_s, ca_d, cb_s, cb_d = (int(x) for x in counts)
+    return m.nb * (ca_s + ca_d) + m.na * (cb_s + cb_d) + ca_s * cb_s
+
+
+def time_sigma(m, hii, T, steps, warmup=2):
+    """Average milliseconds of one FP64 sigma launch (HIP events on the stream
+    the launches go to)."""
+    import torch
+    Wb = torch.empty_like(T)
+    for _ in range(warmup):
+        m.run(hii, T, Wb, accumulate=False)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        m.run(hii, T, Wb, accumulate=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    del Wb
+    return e0.elapsed_time(e1) / steps
+
+
+def time_to_energy(hist, Eref, tol=1e-6):
+    """Seconds until an FP64 Ritz value lies within tol of Eref (history rows
+    (E0, |R|, seconds, precision))."""
+    for E0, _, t, p in hist:
+        if p == 64 and abs(E0 - Eref) <= tol:
+            return round(float(t), 4)
+    return None
+
+
+def _plain_copy(tmp, rel):
+    out = os.path.join(tmp, os.path.basename(rel)[:-3])
+    with gzip.open(os.path.join(GOLDEN, rel), "rb") as f, open(out, "wb") as g:
+        shutil.copyfileobj(f, g)
+    return out
+
+
+# the reference's inputs and the energies its READMEs hold (8 decimals)
+MOLECULES = [
+    ("c2h2_100k", "c2h2/fcidump.txt.gz", "c2h2/alpha-100k.txt.gz", -77.11337996,
+     "data/c2h2/README.md:7"),
+    ("h2o_1em6", "h2o/fcidump.txt.gz", "h2o/h2o-1em6-alpha.txt.gz", -76.24377593,
+     "data/h2o/README.md:11"),
+]
+
+
+def dense_section(args):
+    """Config 1 (FCI norb 12) on the GPU and on the host, and the reference's
+    larger molecule inputs against their held energies."""
+    import numpy as np
+    import torch
+    import sbd_amd
+    from sbd_amd import synth
+    out = {}
+    w = synth.make_workload("fci12", 1.0)
+    m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
+    counts = m.table_counts()
+    terms = terms_per_sigma(m, counts)
+    hii = m.hdiag()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(12)
+    T = torch.randn(m.N, dtype=torch.float64, device="cuda", generator=g)
+    ms = time_sigma(m, hii, T, 20)
+    gold = json.load(open(os.path.join(GOLDEN, "energies.json")))["fci12"]
+    opt = sbd_amd.SBD(max_it=100, max_nb=10, eps=1e-8, init=gold["init"], seed=gold["seed"])
+    E, _, rep, hist = m.diag(opt, history=True)
+    fci = {"determinants": m.N, "terms_per_det": round(terms / m.N, 1),
+           "sigma_ms": round(ms, 4), "sigma_gdet_s": round(m.N / ms * 1e-6, 3),
+           "sigma_gterms_s": round(terms / ms * 1e-6, 2),
+           "energy": E, "abs_err_vs_oracle": abs(E - gold["energy"]),
+           "steps": rep["steps"], "converged": bool(rep["converged"]),
+           "t_davidson_s": round(rep["t_davidson"], 4),
+           "time_to_energy_s": time_to_energy(hist, gold["energy"]), "eps": 1e-8,
+           "bound": "L2/VALU (the alpha x beta pass gathers integrals per term), not HBM"}
+    log(f"dense fci12: sigma {ms:.3f} ms ({fci['sigma_gterms_s']} Gterms/s), "
+        f"E={E:.12f} |dE|={fci['abs_err_vs_oracle']:.1e} steps={rep['steps']} "
+        f"t={rep['t_davidson']:.3f}s")
+    if not args.no_cpu:
+        # the oracle's sigma and Davidson on the host, same inputs and options
+        from oracle import oracle as O
+        S = O.System.from_workload(w)
+        h = S.hdiag()
+        X = T.cpu().numpy()
+        t0 = time.time()
+        Y = S.sigma(h, X)
+        t_sig = time.time() - t0
+        exact = bool(np.array_equal(Y, m.sigma(hii, T).cpu().numpy()))
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        fci["cpu_oracle"] = {"sigma_s": round(t_sig, 4), "sigma_gdet_s": round(S.N / t_sig / 1e9, 5),
+                             "cores": threads, "kind": "port", "sigma_bitexact_vs_gpu": exact}
+        # the oracle's Davidson makes one sigma per step, as the GPU solve did:
+        # run it whole when that fits the time allowed, else report the estimate
+        est = t_sig * (rep["steps"] + 1)
+        if est <= args.cpu_davidson_s:
+            W0 = synth.randomize(S.N, gold["seed"])
+            t0 = time.time()
+            Wc, hc = S.davidson(h, W0, 100, 10, 1e-8)
+            t_dav = time.time() - t0
+            fci["cpu_oracle"].update({"davidson_s": round(t_dav, 3), "davidson_steps": len(hc),
+                                      "davidson_energy": S.energy(h, Wc)})
+            log(f"dense fci12 cpu oracle: sigma {t_sig:.3f}s, Davidson {t_dav:.2f}s "
+                f"({len(hc)} steps), bit-exact sigma {exact}")
+        else:
+            fci["cpu_oracle"]["davidson_s_estimate"] = round(est, 1)
+            log(f"dense fci12 cpu oracle: sigma {t_sig:.3f}s; Davidson skipped (~{est:.0f}s)")
+    out["fci12"] = fci
+    m.close()
+    del T, hii
+    if args.no_molecules:
+        return out
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, fd_rel, a_rel, E_held, cite in MOLECULES:
+            t0 = time.time()
+            fd = sbd_amd.LoadFCIDump(_plain_copy(tmp, fd_rel))
+            a = sbd_amd.LoadAlphaDets(_plain_copy(tmp, a_rel), fd.norb)
+            m = sbd_amd.MultTPB(fd.norb, a, a, fcidump=fd)
+            t_setup = time.time() - t0
+            counts = m.table_counts()
+            terms = terms_per_sigma(m, counts)
+            hii = m.hdiag()
+            T = torch.randn(m.N, dtype=torch.float64, device="cuda", generator=g)
+            ms = time_sigma(m, hii, T, 2, warmup=1)
+            del T, hii
+            E, _, rep, hist = m.diag(sbd_amd.SBD(max_it=60, max_nb=10, eps=1e-6), history=True)
+            out[name] = {"determinants": m.N, "norb": fd.norb, "strings": len(a),
+                         "terms_per_det": round(terms / m.N, 1), "sigma_ms": round(ms, 3),
+                         "sigma_gdet_s": round(m.N / ms * 1e-6, 3),
+                         "sigma_gterms_s": round(terms / ms * 1e-6, 2),
+                         "energy": E, "energy_held": E_held, "held_at": cite,
+                         "abs_err_vs_held": abs(E - E_held), "steps": rep["steps"],
+                         "converged": bool(rep["converged"]),
+                         "t_davidson_s": round(rep["t_davidson"], 3),
+                         "setup_s": round(t_setup, 2), "eps": 1e-6}
+            log(f"dense {name}: N={m.N:.3e} sigma {ms:.2f} ms, E={E:.10f} "
+                f"(held {E_held}) steps={rep['steps']} t={rep['t_davidson']:.2f}s")
+            m.close()
+    return out
+
+
+def precision_section(args):
+    """Config 3 at its size: plain FP64 Davidson against FP32 Davidson + FP64
+    sweep on 3.66e9 determinants, both from the HF determinant (init 0)."""
+    import sbd_amd
+    from sbd_amd import synth
+    t0 = time.time()
+    w = synth.make_workload("sci58hf", 100.0)
+    m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
+    log(f"precision: sci58 strings + HF singles, {len(w['a'])} x {len(w['b'])} = {m.N:.3e} dets, "
+        f"set up in {time.time() - t0:.1f}s")
+    del w
+    res = {}
+    for label, prec in (("fp64", 0), ("mixed", 1)):
+        opt = sbd_amd.SBD(max_it=args.prec_max_it, max_nb=10, eps=args.prec_eps, init=0,
+                          precision=prec)
+        E, _, rep, hist = m.diag(opt, history=True)
+        res[label] = (E, rep, hist)
+        log(f"precision {label}: E={E:.12f} steps={rep['steps']} (fp32 {rep['steps_fp32']}) "
+            f"nb {rep['nb_used']}/{rep['nb_used_fp32']} conv={rep['converged']} "
+            f"t={rep['t_davidson']:.2f}s")
+    Eref = res["fp64"][0]
+    out = {"workload": "norb=58 na=nb=10, sci58's 60000 x 60000 sampled strings plus the 480 "
+                       "HF singles per spin, ERI rank 116, init 0 (HF start)",
+           "determinants": m.N, "eps": args.prec_eps}
+    for label, (E, rep, hist) in res.items():
+        out[label] = {"energy": E, "abs_err_vs_fp64": abs(E - Eref),
+                      "time_to_energy_s": time_to_energy(hist, Eref),
+                      "t_davidson_s": round(rep["t_davidson"], 3), "steps": rep["steps"],
+                      "steps_fp32": rep["steps_fp32"], "converged": bool(rep["converged"]),
+                      "residual": rep["residual"], "subspace_fp64": rep["nb_used"],
+                      "subspace_fp32": rep["nb_used_fp32"]}
+    t64, tmx = out["fp64"]["time_to_energy_s"], out["mixed"]["time_to_energy_s"]
+    out["speedup_mixed_vs_fp64"] = round(t64 / tmx, 3) if t64 and tmx else None
+    m.close()
+    return out
+
+
+def multiroot_section(args):
+    """Config 5, reduced: concurrent block-Davidson requests for the 4 lowest
+    roots (sampled strings as configured, started on the lowest diagonal
+    elements), served in
+    FP32 (FP64 Ritz and Gram-Schmidt, FP64 finish) and again in FP64; per-root
+    energy error of the FP32 results against the FP64 ones."""
+    import numpy as np
+    import sbd_amd
+    from sbd_amd import synth
+    reqs = synth.multiroot_requests(args.mr_count, 5151, args.mr_scale, hf=False)
+    dets = [len(r["a"]) * len(r["b"]) for r in reqs]
+    log(f"multiroot: {len(reqs)} requests, {min(dets):.2e}..{max(dets):.2e} dets")
+    out = {"requests": len(reqs), "nroots": 4, "string_scale": args.mr_scale, "jitter": 0.3,
+           "determinants_min": min(dets), "determinants_max": max(dets)}
+    got = {}
+    for label, prec in (("fp32", 1), ("fp64", 0)):
+        # init 0: unit vectors on the lowest diagonal elements (from random
+        # vectors the diagonal preconditioner returns corrections nearly
+        # parallel to the iterate on these nearly diagonal bases)
+        opt = sbd_amd.SBD(max_it=args.mr_max_it, max_nb=24, eps=1e-6, init=0, precision=prec,
+                          fp32_eps=1e-4)
+        sv = sbd_amd.Server(args.serve_mem_gb << 30, max_active=args.serve_slots)
+        ids = [sv.submit(r["norb"], r["a"], r["b"], I0=r["I0"], h=r["h"], eri=r["eri"], sbd=opt,
+                         nroots=4) for r in reqs]
+        res = []
+        for i in ids:
+            res.append(sv.wait(i))
+            if len(res) % 8 == 0:
+                log(f"multiroot {label}: {len(res)} solves done")
+        sv.close()
+        lat = [r["t_done"] - r["t_submit"] for r in res]
+        span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        got[label] = res
+        out[label] = {"solves_per_s": round(len(res) / span, 3), "p50_s": round(pct(lat, 50), 4),
+                      "p99_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3),
+                      "converged": sum(r["converged"] for r in res)}
+        log(f"multiroot {label}: p50 {pct(lat, 50):.2f}s p99 {pct(lat, 99):.2f}s "
+            f"{len(res) / span:.2f} solves/s, converged {out[label]['converged']}/{len(res)}")
+    err = np.array([[abs(a - b) for a, b in zip(x["energies"], y["energies"])]
+                    for x, y in zip(got["fp32"], got["fp64"])])
+    out["abs_err_fp32_vs_fp64_per_root_max"] = [float(v) for v in err.max(axis=0)]
+    out["abs_err_fp32_vs_fp64_per_root_median"] = [float(v) for v in np.median(err, axis=0)]
+    return out
 
 
 def pct(xs, q):
@@ -181,9 +426,10 @@ def serving(args):
     opt = sbd_amd.SBD(max_it=args.serve_max_it, max_nb=10, eps=args.serve_eps, init=0)
     out = {"options": {"max_it": args.serve_max_it, "max_nb": 10, "eps": args.serve_eps,
                        "init": 0}}
+    pol = tuple(args.serve_policies.split(","))
     for label, count, rate, seed, policies in (
-            ("overload", args.serve_count, args.serve_rate, 4004, ("fifo", "sjf")),
-            ("steady", args.serve_steady_count, args.serve_steady_rate, 4005, ("sjf",))):
+            ("overload", args.serve_count, args.serve_rate, 4004, pol),
+            ("steady", args.serve_steady_count, args.serve_steady_rate, 4005, pol)):
         if count <= 0:
             continue
         t0 = time.time()
@@ -209,18 +455,35 @@ def main():
                     help="Davidson max_it (restarts of max_nb=10 steps each)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-rdm", action="store_true")
-    ap.add_argument("--serve-count", type=int, default=64,
-                    help="requests in the stream (the serving configuration names 256)")
+    ap.add_argument("--serve-count", type=int, default=256,
+                    help="requests in the overload stream")
     ap.add_argument("--serve-rate", type=float, default=40.0,
                     help="Poisson arrivals per second of the overload stream")
-    ap.add_argument("--serve-steady-count", type=int, default=32,
+    ap.add_argument("--serve-steady-count", type=int, default=0,
                     help="requests in the steady (sub-saturation) stream; 0 skips it")
     ap.add_argument("--serve-steady-rate", type=float, default=0.5,
                     help="Poisson arrivals per second of the steady stream")
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-mem-gb", type=int, default=160)
     ap.add_argument("--serve-eps", type=float, default=1e-6)
-    ap.add_argument("--serve-max-it", type=int, default=30)
+    ap.add_argument("--serve-max-it", type=int, default=20)
+    ap.add_argument("--serve-policies", default="sjf",
+                    help="admission policies of the batched runs, comma-separated (fifo, sjf)")
+    ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--cpu-davidson-s", type=float, default=60.0,
+                    help="longest host Davidson (oracle, FCI norb 12) the dense section runs")
+    ap.add_argument("--no-molecules", action="store_true",
+                    help="dense section without the reference's molecule inputs")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--prec-max-it", type=int, default=80,
+                    help="Davidson restarts of the precision section (subspace 10, or what fits)")
+    ap.add_argument("--prec-eps", type=float, default=1e-6,
+                    help="residual of the precision section's solves")
+    ap.add_argument("--no-multiroot", action="store_true")
+    ap.add_argument("--mr-count", type=int, default=32)
+    ap.add_argument("--mr-scale", type=float, default=0.1,
+                    help="string-count scale of the multi-root requests (1.0 = 30000 x 25000)")
+    ap.add_argument("--mr-max-it", type=int, default=60)
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -373,8 +636,17 @@ def main():
         del x, one, two
     m.close()
     del T, Wb, hii
-    if not args.no_serving:
-        out["serving"] = serving(args)
+    torch.cuda.empty_cache()
+    for name, skip, fn in (("dense", args.no_dense, dense_section),
+                           ("precision", args.no_precision, precision_section),
+                           ("serving", args.no_serving, serving),
+                           ("multiroot", args.no_multiroot, multiroot_section)):
+        if skip:
+            continue
+        t0 = time.time()
+        out[name] = fn(args)
+        torch.cuda.empty_cache()
+        log(f"section {name} done in {time.time() - t0:.1f}s")
     print(json.dumps(out), flush=True)
 
 

@@ -282,3 +282,30 @@ def serving_requests(count: int, seed: int, norb=(16, 40), nelec=(4, 10), string
         out.append(dict(id=q, norb=L, n=n, I0=I0, h=h, eri=eri, t=t,
                         a=random_strings(L, n, ca, s0 + 1), b=random_strings(L, n, cb, s0 + 2)))
     return out
+
+
+def multiroot_requests(count: int, seed: int, scale: float = 0.1, jitter: float = 0.3,
+                       workload: str = "sci48open", hf: bool = True):
+    """Concurrent multi-root requests on one workload's integrals: request q
+    draws its alpha and beta string counts as the workload's counts x scale x
+    (1 + jitter u), u uniform in [-1, 1), random weight-n strings from a
+    per-request seed, plus (hf) every single excitation of the Hartree-Fock
+    string, so that each basis holds a connected HF component."""
+    w = WORKLOADS[workload]
+    L = w["norb"]
+    I0, h, eri = make_integrals(L, w["seed"], w["rank"])
+    rng = SplitMix64(seed)
+    out = []
+    for q in range(count):
+        r = rng.draw(4)
+        u = (r >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+        ca = max(2, int(round(w["ca"] * scale * (1.0 + jitter * (2.0 * u[0] - 1.0)))))
+        cb = max(2, int(round(w["cb"] * scale * (1.0 + jitter * (2.0 * u[1] - 1.0)))))
+        s0 = int(r[2] >> np.uint64(1))
+        a = random_strings(L, w["na"], ca, s0 + 1)
+        b = random_strings(L, w["nb"], cb, s0 + 2)
+        if hf:
+            a, b = (np.array(sorted(set(x.tolist()) | set(hf_singles(L, n).tolist())),
+                             dtype=np.uint64) for x, n in ((a, w["na"]), (b, w["nb"])))
+        out.append(dict(id=q, norb=L, I0=I0, h=h, eri=eri, a=a, b=b))
+    return out
