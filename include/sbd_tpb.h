@@ -152,9 +152,14 @@ void sbd_ctx_destroy(sbd_ctx* ctx);
  * others the row kernel, whose alpha-single x beta-single terms go to a pass
  * of their own in bases with at least 4 of them per determinant; 1: row
  * kernel (with that pass where it applies); 2: per-element kernel; 3: row
- * kernel with every term inline.  All give the same bits, and the serving
- * engine's batches follow the same rule. */
+ * kernel with every term inline; 4: split form -- a streaming kernel for the
+ * diagonal and alpha terms over column slabs, then a row kernel for the beta
+ * and alpha-single x beta-single terms (bases whose bra rows fit LDS).  All
+ * give the same bits, and the serving engine's batches follow the same rule. */
 int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel);
+/* Slab width (columns: 256, 512 or 1024) of the split form's streaming kernel
+ * (no counterpart in the reference: a tuning knob; results do not depend on it). */
+int sbd_set_sigma_slab(sbd_ctx* ctx, int cols);
 
 /* Dimensions: out[0] = na, out[1] = nb, out[2] = N = na*nb, out[3] = norb. */
 int sbd_ctx_dims(const sbd_ctx* ctx, uint64_t* out);
@@ -238,6 +243,17 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
 int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, double* energy,
                   double* W_out, double* hist, int hist_cap, sbd_report* rep);
 
+/* sbd_diag from the unit vector on the determinant (a_string, b_string) (one
+ * uint64 per string, orbital p -> bit p; opt->init is ignored): the explicit
+ * initial determinant of sbd::tpb::diag (initial_adeterminant_bitstring /
+ * initial_bdeterminant_bitstring, tpb/sbdiag.h:303-335) through
+ * BasisInitVectorFromDeterminants (tpb/davidson.h:89-127).  SBD_ERR_ARG with
+ * "initial alpha (beta) determinant is not present in the basis" when a
+ * string is not one of the context's. */
+int sbd_diag_from_det(sbd_ctx* ctx, const sbd_options* opt, uint64_t a_string, uint64_t b_string,
+                      double* energy, double* W_out, double* hist, int hist_cap,
+                      sbd_report* rep);
+
 /* Wavefunction files (tpb/restart.h:20-65, 414-657) at one rank: the file is
  * prefix + "000000" and holds size_t na, nb, words per string, the alpha then
  * beta strings as det_vector words of bit_length bits, then W (host, N
@@ -248,6 +264,16 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
 int sbd_save_wavefunction(sbd_ctx* ctx, const char* prefix, size_t bit_length, const double* W);
 int sbd_load_wavefunction(sbd_ctx* ctx, const char* prefix, size_t bit_length, double* W,
                           uint64_t* placed);
+
+/* SaveMatrixFormWF (tpb/restart.h:663-780, called for dump_matrix_form_wf at
+ * tpb/sbdiag.h:808-813) at one rank, from host arrays: by the file's extension
+ * (get_extension, framework/bit_manipulation.h:1145-1166), "bin" writes the
+ * na * nb doubles of W as they lie; "dat" or "txt" writes one line per
+ * determinant, "<W> # <ia>: <alpha bits> <ib>: <beta bits>" (W as a default
+ * ostream writes a double, strings as makestring: orbital norb - 1 first);
+ * any other extension writes nothing.  Strings are one uint64 each. */
+int sbd_save_matrix_form_wf(const char* filename, int norb, const uint64_t* a, size_t na,
+                            const uint64_t* b, size_t nb, const double* W);
 
 /* The nroots lowest states by block Davidson (Davidson-Liu, restarted GD+k
  * style from the current and previous Ritz vectors): sbd::tpb::diag

@@ -30,12 +30,13 @@ _LIB = None
 EXPORTS = [
     "sbd_status_string", "sbd_last_error", "sbd_options_default", "sbd_pack_strings",
     "sbd_fcidump_read", "sbd_fcidump_entries", "sbd_ctx_create_fcidump", "sbd_ctx_create",
-    "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_set_sigma_kernel", "sbd_make_helpers",
+    "sbd_ctx_destroy", "sbd_ctx_dims", "sbd_set_sigma_kernel", "sbd_set_sigma_slab",
+    "sbd_make_helpers",
     "sbd_table_counts", "sbd_table_get",
     "sbd_hdiag", "sbd_hdiag_f32", "sbd_sigma", "sbd_sigma_f32", "sbd_inner", "sbd_inner_noalloc",
     "sbd_davidson",
     "sbd_lanczos", "sbd_string_weights", "sbd_occupation", "sbd_correlation", "sbd_carryover",
-    "sbd_diag_from",
+    "sbd_diag_from", "sbd_diag_from_det", "sbd_save_matrix_form_wf",
     "sbd_save_wavefunction", "sbd_load_wavefunction", "sbd_singles_extend",
     "sbd_singles_extend_strings", "sbd_sort_strings", "sbd_decode_strings_bin",
     "sbd_encode_strings_bin",
@@ -115,6 +116,7 @@ def lib():
     L.sbd_ctx_destroy.argtypes = [vp]
     L.sbd_ctx_dims.argtypes = [vp, up]
     L.sbd_set_sigma_kernel.argtypes = [vp, C.c_int]
+    L.sbd_set_sigma_slab.argtypes = [vp, C.c_int]
     L.sbd_make_helpers.argtypes = [vp]
     L.sbd_table_counts.argtypes = [vp, up]
     L.sbd_table_get.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -136,6 +138,10 @@ def lib():
     L.sbd_encode_strings_bin.argtypes = [C.c_char_p, C.c_int, up, C.c_size_t]
     L.sbd_diag_from.argtypes = [vp, C.POINTER(Options), dp, dp, dp, dp, C.c_int,
                                 C.POINTER(Report)]
+    L.sbd_diag_from_det.argtypes = [vp, C.POINTER(Options), C.c_uint64, C.c_uint64, dp, dp, dp,
+                                    C.c_int, C.POINTER(Report)]
+    L.sbd_save_matrix_form_wf.argtypes = [C.c_char_p, C.c_int, up, C.c_size_t, up, C.c_size_t,
+                                          dp]
     L.sbd_save_wavefunction.argtypes = [vp, C.c_char_p, C.c_size_t, dp]
     L.sbd_load_wavefunction.argtypes = [vp, C.c_char_p, C.c_size_t, dp, up]
     L.sbd_occupation.argtypes = [vp, vp, dp, vp]
@@ -358,10 +364,17 @@ class MultTPB:
     def set_sigma_kernel(self, kernel: str = "auto"):
         """Sigma kernel: "auto" (by sparsity), "rows" (row kernel; dense
         alpha-single x beta-single terms in a pass of their own), "elements"
-        (one element per thread) or "rows_inline" (row kernel, every term
-        inline); all give the same results."""
-        k = {"auto": 0, "rows": 1, "elements": 2, "rows_inline": 3}[kernel]
+        (one element per thread), "rows_inline" (row kernel, every term
+        inline) or "split" (a streaming kernel for the diagonal and alpha
+        terms over column slabs, then a row kernel for the beta terms); all
+        give the same results."""
+        k = {"auto": 0, "rows": 1, "elements": 2, "rows_inline": 3, "split": 4}[kernel]
         _check(lib().sbd_set_sigma_kernel(self._ctx, k), self._ctx)
+
+    def set_sigma_slab(self, cols: int):
+        """Slab width (256, 512 or 1024 columns) of the split sigma's streaming
+        kernel; results do not depend on it."""
+        _check(lib().sbd_set_sigma_slab(self._ctx, int(cols)), self._ctx)
 
     def table_counts(self):
         out = np.zeros(4, dtype=np.uint64)

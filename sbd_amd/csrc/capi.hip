@@ -427,7 +427,8 @@ namespace sbdx {
 // wavefunction, sbdiag.h:339-347), else e_0 (init 0) or the seeded Randomize
 // vector (init 1; seed 0 draws a seed from the clock, as the reference does).
 template <typename VT>
-int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT* W) {
+int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT* W,
+                 int64_t unit) {
   const uint64_t N = ctx->N;
   hipStream_t st = ctx->stream;
   int rc = SBD_OK;
@@ -447,10 +448,13 @@ int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT*
         if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = SBD_ERR_DEVICE;
       }
     }
-  } else if (opt->init == 0) {
+  } else if (unit >= 0 || opt->init == 0) {
+    // e_unit (the explicit initial determinant, davidson.h:89-127) or e_0
+    // (init 0, davidson.h:156-166)
     rc = vec_fill<VT>(ctx, W, N, 0.0, st);
     const VT one = (VT)1;
-    if (!rc) SBD_HIP_TRY(ctx, hipMemcpyAsync(W, &one, sizeof(VT), hipMemcpyHostToDevice, st));
+    const uint64_t k = unit >= 0 ? (uint64_t)unit : 0;
+    if (!rc) SBD_HIP_TRY(ctx, hipMemcpyAsync(W + k, &one, sizeof(VT), hipMemcpyHostToDevice, st));
   } else {
     uint64_t seed = opt->seed;
     if (seed == 0) {
@@ -464,8 +468,8 @@ int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT*
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = SBD_ERR_DEVICE;
   return rc;
 }
-template int start_vector<double>(sbd_ctx*, const sbd_options*, const double*, double*);
-template int start_vector<float>(sbd_ctx*, const sbd_options*, const double*, float*);
+template int start_vector<double>(sbd_ctx*, const sbd_options*, const double*, double*, int64_t);
+template int start_vector<float>(sbd_ctx*, const sbd_options*, const double*, float*, int64_t);
 }  // namespace sbdx
 
 extern "C" {
@@ -640,8 +644,14 @@ void sbd_ctx_destroy(sbd_ctx* ctx) {
 }
 
 int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel) {
-  if (!ctx || kernel < 0 || kernel > 3) return SBD_ERR_ARG;
+  if (!ctx || kernel < 0 || kernel > 4) return SBD_ERR_ARG;
   ctx->sigma_kernel = kernel;
+  return SBD_OK;
+}
+
+int sbd_set_sigma_slab(sbd_ctx* ctx, int cols) {
+  if (!ctx || (cols != 256 && cols != 512 && cols != 1024)) return SBD_ERR_ARG;
+  ctx->stream_cols = cols;
   return SBD_OK;
 }
 
@@ -811,8 +821,33 @@ int sbd_diag(sbd_ctx* ctx, const sbd_options* opt, double* energy, double* W_out
   return sbd_diag_from(ctx, opt, nullptr, energy, W_out, hist, hist_cap, rep);
 }
 
+static int diag_impl(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, int64_t unit,
+                     double* energy, double* W_out, double* hist, int hist_cap, sbd_report* rep);
+
 int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, double* energy,
                   double* W_out, double* hist, int hist_cap, sbd_report* rep) {
+  return diag_impl(ctx, opt, W_init, -1, energy, W_out, hist, hist_cap, rep);
+}
+
+int sbd_diag_from_det(sbd_ctx* ctx, const sbd_options* opt, uint64_t a_string, uint64_t b_string,
+                      double* energy, double* W_out, double* hist, int hist_cap,
+                      sbd_report* rep) {
+  if (!ctx || !opt) return SBD_ERR_ARG;
+  // unique_index of BasisInitVectorFromDeterminants (davidson.h:97-110): the
+  // strings of a context are sorted and unique, so a match is unique
+  auto find = [](const std::vector<uint64_t>& v, uint64_t x) -> int64_t {
+    auto it = std::lower_bound(v.begin(), v.end(), x);
+    return (it != v.end() && *it == x) ? (int64_t)(it - v.begin()) : -1;
+  };
+  const int64_t ia = find(ctx->a_host, a_string), ib = find(ctx->b_host, b_string);
+  if (ia < 0) return set_err(ctx, SBD_ERR_ARG, "initial alpha determinant is not present in the basis");
+  if (ib < 0) return set_err(ctx, SBD_ERR_ARG, "initial beta determinant is not present in the basis");
+  return diag_impl(ctx, opt, nullptr, ia * (int64_t)ctx->nb + ib, energy, W_out, hist, hist_cap,
+                   rep);
+}
+
+static int diag_impl(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, int64_t unit,
+                     double* energy, double* W_out, double* hist, int hist_cap, sbd_report* rep) {
   if (!ctx || !opt) return SBD_ERR_ARG;
   std::vector<double> hv;
   std::vector<double>* H = hist ? &hv : nullptr;
@@ -850,7 +885,7 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
       DevBuf<float> hii32;
       if (hipMalloc(&hii32.p, 4 * N) != hipSuccess || hipMalloc(&W32.p, 4 * N) != hipSuccess)
         return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-      rc = start_vector<float>(ctx, opt, W_init, W32.p);
+      rc = start_vector<float>(ctx, opt, W_init, W32.p, unit);
       if (rc) return rc;
       const double th = now_s();
       rc = launch_hdiag_f32(ctx, hii32.p, st);
@@ -880,7 +915,7 @@ int sbd_diag_from(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, do
   } else {
     if (hipMalloc(&W.p, 8 * N) != hipSuccess)
       return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-    rc = start_vector<double>(ctx, opt, W_init, W.p);
+    rc = start_vector<double>(ctx, opt, W_init, W.p, unit);
     if (rc) return rc;
   }
   if (hipMalloc(&hii.p, 8 * N) != hipSuccess)

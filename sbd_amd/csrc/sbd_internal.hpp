@@ -153,10 +153,12 @@ struct alignas(16) ColRec {
   uint32_t s0, s1;
 };
 
-// Column (string) at one block-sorted ELL position and its doubles count.
-struct alignas(8) EllPos {
+// Column (string) at one block-sorted ELL position: its doubles count and the
+// range [s0, s1) of its single-excitation records.
+struct alignas(16) EllPos {
   uint32_t col;
   uint32_t cnt;
+  uint32_t s0, s1;
 };
 
 // One spin's excitation tables on the device (CSR by bra string).
@@ -234,7 +236,8 @@ struct sbd_ctx {
   // stream of the solver passes (Davidson, Lanczos, block Davidson, diag):
   // the null stream unless a caller such as the serving engine sets its own
   hipStream_t stream = nullptr;
-  int sigma_kernel = 0;  // 0 = by sparsity, 1 = row kernel, 2 = per-element, 3 = rows, alpha x beta inline
+  int sigma_kernel = 0;  // 0 = by sparsity, 1 = row kernel, 2 = per-element, 3 = rows, alpha x beta inline, 4 = split
+  int stream_cols = 512;  // slab width of the split sigma's streaming kernel (256, 512, 1024)
   int ab_pass = -1;      // dense alpha x beta pass: -1 not yet decided, 0 no, 1 yes
   size_t ab_lds = 0;     // its LDS bytes per workgroup
   uint32_t* d_ab_sell = nullptr;  // its column-major packed beta single records
@@ -293,10 +296,12 @@ template <typename VT>
 int vec_fill(sbd_ctx* ctx, VT* x, uint64_t n, double v, hipStream_t st);
 
 // W[0, N) = the start vector of sbd_diag in W's element type, on ctx->stream
-// (synchronised): the caller's W_init (host FP64) if given, else
+// (synchronised): the caller's W_init (host FP64) if given, else the unit
+// vector e_unit when unit >= 0 (BasisInitVectorFromDeterminants), else
 // BasisInitVector's init 0 (e_0) or init 1 (Randomize with opt->seed).
 template <typename VT>
-int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT* W);
+int start_vector(sbd_ctx* ctx, const sbd_options* opt, const double* W_init, VT* W,
+                 int64_t unit = -1);
 
 // Seconds on a steady clock; records msg as the context's last error and
 // returns code.

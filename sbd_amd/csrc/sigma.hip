@@ -685,6 +685,353 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* _
   sigma_rows_body<VT, true, true, kRows, true>(J.p, blockIdx.x - J.blk0, J.ngrp, J.dot_part);
 }
 
+// ---------------------------------------------------------------- split sigma
+//
+// The row kernel sweeps whole rows, so the alpha ket rows T[ja, :] that a bra
+// row reads come back from HBM once per pair: seven times per element at the
+// bench's size, with no reuse in between (the alpha connectivity of random
+// strings has no locality).  The split sigma cuts the same sum at the same
+// point into two kernels:
+//
+//   k_sigma_stream  diagonal, alpha singles, alpha doubles, for tiles of one
+//                   bra row x kStreamBlock * J consecutive columns, taken
+//                   slab-major (every row of one column slab, then the next
+//                   slab).  The rows of T touched while a slab is in flight
+//                   are that slab's columns only (nα x slab width elements),
+//                   so the seven ket reads of an element find their lines in
+//                   the memory-side cache instead of HBM;
+//   k_sigma_bd      beta singles, beta doubles and alpha-single x beta-single
+//                   terms (sigma_phase2 with all sixteen wavefronts), bra rows
+//                   staged in LDS, reading the partial sums the first kernel
+//                   stored in W.
+//
+// Per element the terms are added in the same order with the same operations
+// as in k_sigma_rows, so the results are the same bits.
+constexpr int kStreamBlock = 256;
+
+// tiles t0, t0 + stride, ... below ntiles of one sigma
+template <typename VT, bool kCache, int J>
+__device__ inline void sigma_stream_body(const SigmaArgs<VT>& p, uint64_t t0, uint64_t stride,
+                                         uint64_t ntiles) {
+  using AT = VT;
+  constexpr int SW = kStreamBlock * J;
+  const uint32_t nb = (uint32_t)p.nb;
+  const int L = p.L;
+  for (uint64_t tile = t0; tile < ntiles; tile += stride) {
+    const uint64_t slab = tile / p.na;
+    const uint64_t ia = tile - slab * p.na;
+    const uint32_t base = (uint32_t)slab * SW;
+    const uint64_t A = p.a[ia];
+    uint32_t col[J];
+    bool ok[J];
+    uint64_t B[J];
+    AT acc[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      const uint32_t c = base + threadIdx.x + j * kStreamBlock;
+      ok[j] = c < nb;
+      col[j] = ok[j] ? c : nb - 1;
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      B[j] = p.b[col[j]];
+      const VT hv = p.hii[ia * nb + col[j]];
+      const VT ts = p.T[ia * nb + col[j]];
+      acc[j] = p.accumulate ? (AT)p.W[ia * nb + col[j]] : (AT)0;
+      acc[j] += (AT)hv * (AT)ts;
+    }
+    // alpha singles (task type 2, OneExcite)
+    const uint32_t as1 = p.as_off[ia + 1];
+    for (uint32_t q = p.as_off[ia]; q < as1; q++) {
+      const PairRec rc = p.as_rec[q];
+      const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
+#pragma unroll
+      for (int j = 0; j < J; j++) {
+        const VT tk = Tk[col[j]];
+        const double e = kCache ? p.Ea[(uint64_t)q * nb + col[j]]
+                                : alpha_single_elem(rc, A, B[j], L, p.G);
+        acc[j] += (AT)e * (AT)tk;
+      }
+    }
+    // alpha doubles (task type 2, TwoExcite), ket loads kBatch records deep
+    const uint32_t q1 = p.ad_off[ia + 1];
+    for (uint32_t q0 = p.ad_off[ia]; q0 < q1; q0 += kBatch) {
+      VT tv[kBatch][J];
+      double vv[kBatch];
+      uint64_t mm[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        if (q0 + u < q1) {
+          const PairRec rc = p.ad_rec[q0 + u];
+          const VT* __restrict__ Tk = p.T + (uint64_t)rc.ket * nb;
+#pragma unroll
+          for (int j = 0; j < J; j++) tv[u][j] = Tk[col[j]];
+          vv[u] = rc.val;
+          mm[u] = beta_mask_double(rc.code);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        if (q0 + u < q1) {
+#pragma unroll
+          for (int j = 0; j < J; j++) {
+            const double e = flip_sign(vv[u], (uint32_t)__popcll(B[j] & mm[u]));
+            acc[j] += (AT)e * (AT)tv[u][j];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++)
+      if (ok[j]) p.W[ia * nb + col[j]] = (VT)acc[j];
+  }
+}
+
+template <typename VT, bool kCache, int J>
+__global__ __launch_bounds__(kStreamBlock) void k_sigma_stream(const SigmaArgs<VT> p,
+                                                               uint64_t ntiles) {
+  sigma_stream_body<VT, kCache, J>(p, blockIdx.x, gridDim.x, ntiles);
+}
+
+// Several sigmas' streaming parts in one launch (the serving engine's batch):
+// job j owns workgroups [blk0, blk0 + ngrp) and takes its tiles grid-stride
+// over them.
+template <typename VT>
+struct StreamJob {
+  SigmaArgs<VT> p;
+  uint32_t blk0, ngrp;
+  uint64_t ntiles;
+};
+template <typename VT, int J>
+__global__ __launch_bounds__(kStreamBlock) void k_sigma_stream_batch(
+    const StreamJob<VT>* __restrict__ jobs, int njobs) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const StreamJob<VT>& Jb = jobs[lo];
+  sigma_stream_body<VT, true, J>(Jb.p, blockIdx.x - Jb.blk0, Jb.ngrp, Jb.ntiles);
+}
+
+// Second kernel of the split sigma: one workgroup per group of kRows bra rows
+// (grid-stride), the rows staged in LDS, then blocks of 2048 block-sorted
+// positions, thread t taking positions blk + t + 1024 j (j < 2; their records
+// loaded one block ahead).  Per position and row a block adds to the partial
+// sum from W the beta singles (records from L2, elements from the cache, T
+// from LDS), the beta doubles from the ELL slices BB slots at a time (the
+// first batch is issued before the singles, so the two latencies overlap) and
+// the alpha-single x beta-single terms, then stores the element.  The
+// strings with single excitations come first in each block's order, so few
+// wavefronts wait for them.  dot_part (optional): per-wavefront <T, W> sums
+// at [blockIdx * 16 + wave].
+constexpr int kBdCols = 2;
+constexpr uint32_t kBdSpan = kBdCols * kSigBlock;  // positions per step
+
+template <typename VT, int kRows, bool kPk, bool kCache>
+__device__ inline double bd_block(const SigmaArgs<VT>& p, const VT* __restrict__ trows,
+                                  const uint64_t* A, uint64_t ia0, int nr, uint32_t blk,
+                                  uint32_t t, const EllPos* pc, bool dot, uint32_t zs,
+                                  uint32_t as0, uint32_t as1) {
+  using AT = VT;
+  constexpr int C = kBdCols;
+  constexpr int BB = kBBatch<VT, kRows>;
+  const uint32_t nb = (uint32_t)p.nb;
+  const int L = p.L;
+  double tw = 0.0;
+  bool okp[C];
+  uint32_t i0[C], w[C];
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const uint32_t pos = blk + j * kSigBlock + t;
+    okp[j] = pos < nb;
+    const uint32_t sl = __builtin_amdgcn_readfirstlane((okp[j] ? pos : nb - 1) / kEllSlice);
+    const uint32_t o0 = p.bd_ell_off[sl];
+    w[j] = (p.bd_ell_off[sl + 1] - o0) / kEllSlice;
+    i0[j] = o0 + pos % kEllSlice;
+  }
+  uint32_t wmax = 0;
+#pragma unroll
+  for (int j = 0; j < C; j++) wmax = w[j] > wmax ? w[j] : wmax;
+  // positions without beta terms keep their partial sums as they are in W
+  bool has[C];
+#pragma unroll
+  for (int j = 0; j < C; j++) has[j] = okp[j] && (pc[j].cnt != 0 || pc[j].s1 > pc[j].s0);
+  AT acc[kRows][C];
+#pragma unroll
+  for (int r = 0; r < kRows; r++)
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      acc[r][j] = (r < nr && has[j]) ? (AT)p.W[(ia0 + r) * nb + pc[j].col] : (AT)0;
+  // first batch of slots: issued now, added after the singles
+  uint32_t kk[C][BB];
+  VT vv[C][BB];
+  uint64_t mm[C][BB];
+#pragma unroll
+  for (int j = 0; j < C; j++)
+#pragma unroll
+    for (int u = 0; u < BB; u++)
+      EllOf<VT>::template load<kPk>(p, (uint32_t)u < w[j] ? i0[j] + u * kEllSlice : zs, kk[j][u],
+                                    vv[j][u], mm[j][u]);
+  // beta singles (task type 1, OneExcite)
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const uint32_t s1 = okp[j] ? pc[j].s1 : pc[j].s0;
+    for (uint32_t q = pc[j].s0; q < s1; q++) {
+      const PairRec rc = p.bs_rec[q];
+#pragma unroll
+      for (int r = 0; r < kRows; r++) {
+        if (r >= nr) continue;
+        const double e = kCache ? p.Eb[(ia0 + r) * (uint64_t)p.Sb + q]
+                                : beta_single_elem(rc, A[r], p.b[pc[j].col], L, p.G);
+        acc[r][j] += (AT)e * (AT)trows[(size_t)r * nb + rc.ket];
+      }
+    }
+  }
+  // beta doubles (task type 1, TwoExcite), in slot order; slots past a
+  // slice's width read the zero slot
+  for (uint32_t k0 = 0; k0 < wmax; k0 += BB) {
+    if (k0 > 0) {
+#pragma unroll
+      for (int j = 0; j < C; j++)
+#pragma unroll
+        for (int u = 0; u < BB; u++)
+          EllOf<VT>::template load<kPk>(p, k0 + u < w[j] ? i0[j] + (k0 + u) * kEllSlice : zs,
+                                        kk[j][u], vv[j][u], mm[j][u]);
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; r++) {
+      if (r >= nr) continue;
+#pragma unroll
+      for (int j = 0; j < C; j++) {
+        VT tt[BB];
+#pragma unroll
+        for (int u = 0; u < BB; u++) tt[u] = trows[(size_t)r * nb + kk[j][u]];
+#pragma unroll
+        for (int u = 0; u < BB; u++) {
+          const VT e = flip_sign(vv[j][u], (uint32_t)__popcll(A[r] & mm[j][u]));
+          acc[r][j] += e * tt[u];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; r++) {
+    if (r >= nr) continue;
+    const uint64_t ia = ia0 + r;
+    const uint32_t a0 = __shfl(as0, r, 64), a1 = __shfl(as1, r, 64);
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      if (!okp[j]) continue;
+      // alpha single x beta single (task type 0): rows with alpha singles
+      // (uniform) and strings with beta singles
+      if (!p.skip_ab && a1 > a0 && pc[j].s1 > pc[j].s0) {
+        const uint64_t B = p.b[pc[j].col];
+        const uint64_t D0 = spread32(A[r]) | (spread32(B) << 1);
+        const uint64_t D1 = spread32(A[r] >> 32) | (spread32(B >> 32) << 1);
+        for (uint32_t q = a0; q < a1; q++) {
+          const PairRec ra = p.as_rec[q];
+          const int xa = code_x0(ra.code), ya = code_y0(ra.code);
+          const VT* __restrict__ Tk = p.T + (uint64_t)ra.ket * nb;
+          for (uint32_t k0 = pc[j].s0; k0 < pc[j].s1; k0 += kABatch) {
+            uint32_t code[kABatch];
+            VT tv[kABatch];
+#pragma unroll
+            for (int u = 0; u < kABatch; u++) {
+              const PairRec rb = p.bs_rec[k0 + u < pc[j].s1 ? k0 + u : pc[j].s1 - 1];
+              code[u] = rb.code;
+              tv[u] = Tk[rb.ket];
+            }
+#pragma unroll
+            for (int u = 0; u < kABatch; u++) {
+              const double e =
+                  ab_elem(D0, D1, xa, ya, code_x0(code[u]), code_y0(code[u]), L, p.Vab);
+              const AT nv = acc[r][j] + (AT)e * (AT)tv[u];
+              acc[r][j] = (k0 + u < pc[j].s1) ? nv : acc[r][j];
+            }
+          }
+        }
+      }
+      if (has[j]) p.W[ia * nb + pc[j].col] = (VT)acc[r][j];
+      else if (dot) acc[r][j] = (AT)p.W[ia * nb + pc[j].col];
+      if (dot) tw += (double)trows[(size_t)r * nb + pc[j].col] * (double)(VT)acc[r][j];
+    }
+  }
+  return tw;
+}
+
+// row groups bid, bid + ngrp, ... of one sigma; dot_part (optional) receives
+// this workgroup's per-wavefront <T, W> sums at [bid * 16 + wave]
+template <typename VT, int kRows, bool kPk, bool kCache>
+__device__ inline void sigma_bd_body(const SigmaArgs<VT>& p, uint32_t bid, uint32_t ngrp,
+                                     double* __restrict__ dot_part) {
+  constexpr int C = kBdCols;
+  extern __shared__ unsigned char lds_raw[];
+  VT* trows = reinterpret_cast<VT*>(lds_raw);
+  const uint32_t nb = (uint32_t)p.nb;
+  const uint32_t t = threadIdx.x;
+  const uint32_t zs = p.bd_ell_off[(nb + kEllSlice - 1) / kEllSlice];
+  double tw = 0.0;
+  for (uint64_t ia0 = (uint64_t)bid * kRows; ia0 < p.na; ia0 += (uint64_t)ngrp * kRows) {
+    const int nr = (int)(p.na - ia0 < (uint64_t)kRows ? p.na - ia0 : kRows);  // uniform
+    __syncthreads();  // the previous group's rows are no longer read
+#pragma unroll
+    for (int r = 0; r < kRows; r++)
+      if (r < nr) stage_row<VT>(trows + (size_t)r * nb, p.T + (ia0 + r) * nb, nb);
+    uint64_t A[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; r++) A[r] = r < nr ? p.a[ia0 + r] : 0;
+    // alpha single ranges of the rows: lane r holds row r's (read with __shfl)
+    const uint32_t lr = (uint32_t)(threadIdx.x & 63) < (uint32_t)nr ? (threadIdx.x & 63) : 0;
+    const uint32_t as0 = p.as_off[ia0 + lr], as1 = p.as_off[ia0 + lr + 1];
+    EllPos pc[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      const uint32_t i = j * kSigBlock + t;
+      pc[j] = p.bd_ell_pos[i < nb ? i : nb - 1];
+    }
+    __syncthreads();
+    for (uint32_t blk = 0; blk < nb; blk += kBdSpan) {
+      tw += bd_block<VT, kRows, kPk, kCache>(p, trows, A, ia0, nr, blk, t, pc,
+                                             dot_part != nullptr, zs, as0, as1);
+      if (blk + kBdSpan < nb) {
+#pragma unroll
+        for (int j = 0; j < C; j++) {
+          const uint32_t i = blk + kBdSpan + j * kSigBlock + t;
+          pc[j] = p.bd_ell_pos[i < nb ? i : nb - 1];
+        }
+      }
+    }
+  }
+  if (dot_part) {
+    for (int o = 32; o > 0; o >>= 1) tw += __shfl_down(tw, o, 64);
+    if ((threadIdx.x & 63) == 0) dot_part[bid * (kSigBlock / 64) + threadIdx.x / 64] = tw;
+  }
+}
+
+template <typename VT, int kRows, bool kPk, bool kCache>
+__global__ __launch_bounds__(kSigBlock) void k_sigma_bd(const SigmaArgs<VT> p,
+                                                       double* __restrict__ dot_part) {
+  sigma_bd_body<VT, kRows, kPk, kCache>(p, blockIdx.x, gridDim.x, dot_part);
+}
+
+// Several sigmas' second parts in one launch: job j owns workgroups
+// [blk0, blk0 + ngrp), the grid its sigma has alone, so its results and
+// <T, W> partials are those of its own launch.  Packed, cached tables.
+template <typename VT, int kRows>
+__global__ __launch_bounds__(kSigBlock) void k_sigma_bd_batch(const SigmaJob<VT>* __restrict__ jobs,
+                                                             int njobs) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const SigmaJob<VT>& Jb = jobs[lo];
+  sigma_bd_body<VT, kRows, true, true>(Jb.p, blockIdx.x - Jb.blk0, Jb.ngrp, Jb.dot_part);
+}
+
 // ---------------------------------------------------------------- dense alpha x beta
 //
 // In densely connected bases (tens of single excitations per string, as in
@@ -1143,10 +1490,11 @@ static int sigma_rows_for(const sbd_ctx* ctx, size_t elem) {
 // Which kernel a context's sigma takes.  One rule serves single launches and
 // the serving engine's batches, so a solve computes the same way alone and in
 // a batch: the per-element kernel for sparse bases (fewer than kSparseTerms
-// off-diagonal pairs per alpha string plus per beta string), else the row
-// kernel; sbd_set_sigma_kernel can force either.
+// off-diagonal pairs per alpha string plus per beta string), else the split
+// form or the row kernel (sigma_use_split); sbd_set_sigma_kernel can force
+// any of them.
 static bool sigma_use_elems(const sbd_ctx* ctx) {
-  if (ctx->sigma_kernel == 1 || ctx->sigma_kernel == 3) return false;
+  if (ctx->sigma_kernel == 1 || ctx->sigma_kernel == 3 || ctx->sigma_kernel == 4) return false;
   if (ctx->sigma_kernel == 2) return true;
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
@@ -1157,6 +1505,21 @@ static bool sigma_use_elems(const sbd_ctx* ctx) {
 
 static bool sigma_cached(const sbd_ctx* ctx) {
   return (ctx->ta.n_s == 0 || ctx->d_Ea) && (ctx->tb.n_s == 0 || ctx->d_Eb);
+}
+
+// Whether a context outside the per-element rule takes the split form
+// (k_sigma_stream + k_sigma_bd) rather than the row kernel.  Both need the bra
+// rows in LDS.  By default the split form is taken when two or more rows fit
+// LDS together (measured faster in every such case: FP32 at 20,000 beta
+// strings, FCI norb 12, the reference's H2O); with one row per workgroup the
+// row kernel's two overlapped phases win.  Knob 4 forces it where it fits, 1
+// and 3 never take it.
+template <typename VT>
+static bool sigma_use_split(const sbd_ctx* ctx) {
+  if (sizeof(VT) * ctx->nb > (size_t)ctx->lds_max) return false;
+  if (ctx->sigma_kernel == 4) return true;
+  if (ctx->sigma_kernel != 0) return false;
+  return sigma_rows_for(ctx, sizeof(VT)) >= 2;
 }
 
 // The row kernel leaves the alpha-single x beta-single terms to k_sigma_ab
@@ -1220,74 +1583,124 @@ static int sigma_ab_rows_cfg(sbd_ctx* ctx, size_t elem, int* C, bool* dbuf, size
 int launch_sigma_batch(BatchScratch& sc, const SigmaItem* items, int n, double* tw_out,
                        hipStream_t st) {
   if (n < 1) return SBD_OK;
-  std::vector<int> direct, byrows[3];  // rows 1, 2, 4
+  // jobs by form and rows per workgroup (1, 2, 4); the rest take their own
+  // launch
+  std::vector<int> direct, byrows[3], bysplit[3];
   for (int j = 0; j < n; j++) {
     sbd_ctx* c = items[j].ctx;
     if (!c->tables_built) return SBD_ERR_STATE;
     const bool lds = sizeof(double) * c->nb <= (size_t)c->lds_max;
     const int r = lds ? sigma_rows_for(c, sizeof(double)) : 1;
     const uint64_t groups = (c->na + r - 1) / r;
+    const int ri = r == 4 ? 2 : r == 2 ? 1 : 0;
     if (sigma_use_elems(c) || sigma_ab_pass(c) || !sigma_cached(c) || !lds ||
         !c->tb.ell_packed || groups > kSigMaxGrid)
       direct.push_back(j);
+    else if (sigma_use_split<double>(c))
+      bysplit[ri].push_back(j);
     else
-      byrows[r == 4 ? 2 : r == 2 ? 1 : 0].push_back(j);
+      byrows[ri].push_back(j);
   }
-  // jobs outside the batched form: their own launch (the same kernel, grid
+  // jobs outside the batched forms: their own launch (the same kernels, grids
   // and sums as alone)
   for (int j : direct) {
     int rc = launch_sigma<double>(items[j].ctx, items[j].hii, items[j].T, items[j].W, 0, st,
                                   &tw_out[j]);
     if (rc) return rc;
   }
-  for (int pass = 0; pass < 3; pass++) {
-    const std::vector<int>& sel = byrows[pass];
-    if (sel.empty()) continue;
-    const int R = 1 << pass;
-    std::vector<SigmaJob<double>> jobs(sel.size());
-    std::vector<SumJob> sums(sel.size());
-    size_t parts = 0, shm = 0;
+  // the split form's streaming part of every split job, one launch per slab width
+  for (int J : {1, 2, 4}) {
+    std::vector<StreamJob<double>> sj;
     uint32_t blk = 0;
-    for (size_t q = 0; q < sel.size(); q++) {
-      const SigmaItem& it = items[sel[q]];
-      SigmaJob<double>& J = jobs[q];
-      J.p = sigma_args<double>(it.ctx, it.hii, it.T, it.W, 0);
-      J.ngrp = (uint32_t)((it.ctx->na + R - 1) / R);
-      J.blk0 = blk;
-      blk += J.ngrp;
-      parts += (size_t)J.ngrp * (kSigBlock / 64);
-      shm = std::max(shm, (size_t)R * sizeof(double) * it.ctx->nb);
-    }
-    int rc = batch_reserve(sc, std::max(sizeof(SigmaJob<double>), sizeof(SumJob)) * sel.size(),
-                           parts, sel.size());
+    for (int ri = 0; ri < 3; ri++)
+      for (int j : bysplit[ri]) {
+        const sbd_ctx* c = items[j].ctx;
+        const int cj = c->stream_cols >= 1024 ? 4 : c->stream_cols >= 512 ? 2 : 1;
+        if (cj != J) continue;
+        StreamJob<double> q;
+        q.p = sigma_args<double>(c, items[j].hii, items[j].T, items[j].W, 0);
+        const uint64_t sw = (uint64_t)kStreamBlock * J;
+        q.ntiles = ((c->nb + sw - 1) / sw) * c->na;
+        q.ngrp = (uint32_t)std::min<uint64_t>(q.ntiles, (uint64_t)c->cu_count * 8);
+        q.blk0 = blk;
+        blk += q.ngrp;
+        sj.push_back(q);
+      }
+    if (sj.empty()) continue;
+    int rc = batch_reserve(sc, sizeof(StreamJob<double>) * sj.size(), 0, 0);
     if (rc) return rc;
-    size_t off = 0;
-    for (size_t q = 0; q < sel.size(); q++) {
-      jobs[q].dot_part = sc.d_part + off;
-      sums[q] = SumJob{sc.d_part + off, (uint64_t)jobs[q].ngrp * (kSigBlock / 64), sc.d_out + q};
-      off += (size_t)jobs[q].ngrp * (kSigBlock / 64);
-    }
-    if (hipMemcpyAsync(sc.d_jobs, jobs.data(), sizeof(SigmaJob<double>) * jobs.size(),
+    if (hipMemcpyAsync(sc.d_jobs, sj.data(), sizeof(StreamJob<double>) * sj.size(),
                        hipMemcpyHostToDevice, st) != hipSuccess)
       return SBD_ERR_DEVICE;
-    const SigmaJob<double>* dj = (const SigmaJob<double>*)sc.d_jobs;
-    switch (R) {
-      case 4: hipLaunchKernelGGL((k_sigma_batch<double, 4>), dim3(blk), dim3(kSigBlock), shm, st, dj, (int)jobs.size()); break;
-      case 2: hipLaunchKernelGGL((k_sigma_batch<double, 2>), dim3(blk), dim3(kSigBlock), shm, st, dj, (int)jobs.size()); break;
-      default: hipLaunchKernelGGL((k_sigma_batch<double, 1>), dim3(blk), dim3(kSigBlock), shm, st, dj, (int)jobs.size()); break;
+    const StreamJob<double>* dj = (const StreamJob<double>*)sc.d_jobs;
+    switch (J) {
+      case 4: hipLaunchKernelGGL((k_sigma_stream_batch<double, 4>), dim3(blk), dim3(kStreamBlock), 0, st, dj, (int)sj.size()); break;
+      case 2: hipLaunchKernelGGL((k_sigma_stream_batch<double, 2>), dim3(blk), dim3(kStreamBlock), 0, st, dj, (int)sj.size()); break;
+      default: hipLaunchKernelGGL((k_sigma_stream_batch<double, 1>), dim3(blk), dim3(kStreamBlock), 0, st, dj, (int)sj.size()); break;
     }
     if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
-    if (hipMemcpyAsync(sc.d_fin, sums.data(), sizeof(SumJob) * sums.size(), hipMemcpyHostToDevice,
-                       st) != hipSuccess)
-      return SBD_ERR_DEVICE;
-    hipLaunchKernelGGL(k_sum_fixed_batch, dim3((unsigned)sums.size()), dim3(kSumBlock), 0, st,
-                       (const SumJob*)sc.d_fin);
-    if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
-    if (hipMemcpyAsync(sc.h_out, sc.d_out, sizeof(double) * sel.size(), hipMemcpyDeviceToHost,
-                       st) != hipSuccess)
-      return SBD_ERR_DEVICE;
+    // the job table is read by the launch before the next copy overwrites it
     if (hipStreamSynchronize(st) != hipSuccess) return SBD_ERR_DEVICE;
-    for (size_t q = 0; q < sel.size(); q++) tw_out[sel[q]] = sc.h_out[q];
+  }
+  for (int split = 0; split < 2; split++) {
+    for (int pass = 0; pass < 3; pass++) {
+      const std::vector<int>& sel = split ? bysplit[pass] : byrows[pass];
+      if (sel.empty()) continue;
+      const int R = 1 << pass;
+      std::vector<SigmaJob<double>> jobs(sel.size());
+      std::vector<SumJob> sums(sel.size());
+      size_t parts = 0, shm = 0;
+      uint32_t blk = 0;
+      for (size_t q = 0; q < sel.size(); q++) {
+        const SigmaItem& it = items[sel[q]];
+        SigmaJob<double>& J = jobs[q];
+        J.p = sigma_args<double>(it.ctx, it.hii, it.T, it.W, 0);
+        J.ngrp = (uint32_t)((it.ctx->na + R - 1) / R);
+        J.blk0 = blk;
+        blk += J.ngrp;
+        parts += (size_t)J.ngrp * (kSigBlock / 64);
+        shm = std::max(shm, (size_t)R * sizeof(double) * it.ctx->nb);
+      }
+      int rc = batch_reserve(sc, std::max(sizeof(SigmaJob<double>), sizeof(SumJob)) * sel.size(),
+                             parts, sel.size());
+      if (rc) return rc;
+      size_t off = 0;
+      for (size_t q = 0; q < sel.size(); q++) {
+        jobs[q].dot_part = sc.d_part + off;
+        sums[q] = SumJob{sc.d_part + off, (uint64_t)jobs[q].ngrp * (kSigBlock / 64), sc.d_out + q};
+        off += (size_t)jobs[q].ngrp * (kSigBlock / 64);
+      }
+      if (hipMemcpyAsync(sc.d_jobs, jobs.data(), sizeof(SigmaJob<double>) * jobs.size(),
+                         hipMemcpyHostToDevice, st) != hipSuccess)
+        return SBD_ERR_DEVICE;
+      const SigmaJob<double>* dj = (const SigmaJob<double>*)sc.d_jobs;
+      const int nj = (int)jobs.size();
+      if (split) {
+        switch (R) {
+          case 4: hipLaunchKernelGGL((k_sigma_bd_batch<double, 4>), dim3(blk), dim3(kSigBlock), shm, st, dj, nj); break;
+          case 2: hipLaunchKernelGGL((k_sigma_bd_batch<double, 2>), dim3(blk), dim3(kSigBlock), shm, st, dj, nj); break;
+          default: hipLaunchKernelGGL((k_sigma_bd_batch<double, 1>), dim3(blk), dim3(kSigBlock), shm, st, dj, nj); break;
+        }
+      } else {
+        switch (R) {
+          case 4: hipLaunchKernelGGL((k_sigma_batch<double, 4>), dim3(blk), dim3(kSigBlock), shm, st, dj, nj); break;
+          case 2: hipLaunchKernelGGL((k_sigma_batch<double, 2>), dim3(blk), dim3(kSigBlock), shm, st, dj, nj); break;
+          default: hipLaunchKernelGGL((k_sigma_batch<double, 1>), dim3(blk), dim3(kSigBlock), shm, st, dj, nj); break;
+        }
+      }
+      if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
+      if (hipMemcpyAsync(sc.d_fin, sums.data(), sizeof(SumJob) * sums.size(), hipMemcpyHostToDevice,
+                         st) != hipSuccess)
+        return SBD_ERR_DEVICE;
+      hipLaunchKernelGGL(k_sum_fixed_batch, dim3((unsigned)sums.size()), dim3(kSumBlock), 0, st,
+                         (const SumJob*)sc.d_fin);
+      if (hipGetLastError() != hipSuccess) return SBD_ERR_DEVICE;
+      if (hipMemcpyAsync(sc.h_out, sc.d_out, sizeof(double) * sel.size(), hipMemcpyDeviceToHost,
+                         st) != hipSuccess)
+        return SBD_ERR_DEVICE;
+      if (hipStreamSynchronize(st) != hipSuccess) return SBD_ERR_DEVICE;
+      for (size_t q = 0; q < sel.size(); q++) tw_out[sel[q]] = sc.h_out[q];
+    }
   }
   return SBD_OK;
 }
@@ -1350,6 +1763,28 @@ static unsigned rows_grid(const sbd_ctx* ctx, int* rows_out) {
   if (rows_out) *rows_out = rows;
   return (unsigned)(groups < (uint64_t)kSigMaxGrid ? groups : kSigMaxGrid);
 }
+// The split sigma's two launches (see k_sigma_stream): slab width
+// kStreamBlock * J columns from ctx->stream_cols (256, 512 or 1024).
+template <typename VT, bool kCache>
+static void launch_stream(const sbd_ctx* ctx, const SigmaArgs<VT>& args, hipStream_t st) {
+  const int J = ctx->stream_cols >= 1024 ? 4 : ctx->stream_cols >= 512 ? 2 : 1;
+  const uint64_t sw = (uint64_t)kStreamBlock * J;
+  const uint64_t ntiles = ((ctx->nb + sw - 1) / sw) * ctx->na;
+  const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 1u << 23);
+  if (J == 4) hipLaunchKernelGGL((k_sigma_stream<VT, kCache, 4>), dim3(grid), dim3(kStreamBlock), 0, st, args, ntiles);
+  else if (J == 2) hipLaunchKernelGGL((k_sigma_stream<VT, kCache, 2>), dim3(grid), dim3(kStreamBlock), 0, st, args, ntiles);
+  else hipLaunchKernelGGL((k_sigma_stream<VT, kCache, 1>), dim3(grid), dim3(kStreamBlock), 0, st, args, ntiles);
+}
+template <typename VT, int kRows>
+static void launch_bd(const sbd_ctx* ctx, const SigmaArgs<VT>& args, bool cache, unsigned grid,
+                      size_t shm, double* dpart, hipStream_t st) {
+  const bool pk = ctx->tb.ell_packed;
+  if (pk && cache) hipLaunchKernelGGL((k_sigma_bd<VT, kRows, true, true>), dim3(grid), dim3(kSigBlock), shm, st, args, dpart);
+  else if (pk) hipLaunchKernelGGL((k_sigma_bd<VT, kRows, true, false>), dim3(grid), dim3(kSigBlock), shm, st, args, dpart);
+  else if (cache) hipLaunchKernelGGL((k_sigma_bd<VT, kRows, false, true>), dim3(grid), dim3(kSigBlock), shm, st, args, dpart);
+  else hipLaunchKernelGGL((k_sigma_bd<VT, kRows, false, false>), dim3(grid), dim3(kSigBlock), shm, st, args, dpart);
+}
+
 template <typename VT>
 static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accumulate,
                        hipStream_t st, double* tw_dot, double* part) {
@@ -1363,6 +1798,23 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
   const size_t shm = lds ? rows * row_bytes : 0;
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
+  const bool cache = sigma_cached(ctx);
+  if (sigma_use_split<VT>(ctx)) {
+    SigmaArgs<VT> args = sigma_args<VT>(ctx, hii, T, W, accumulate);
+    args.skip_ab = skip_ab;
+    if (cache) launch_stream<VT, true>(ctx, args, st);
+    else launch_stream<VT, false>(ctx, args, st);
+    SBD_HIP_TRY(ctx, hipGetLastError());
+    switch (rows) {
+      case 8:
+        if constexpr (sizeof(VT) == 4) launch_bd<VT, 8>(ctx, args, cache, grid, shm, dpart, st);
+        break;
+      case 4: launch_bd<VT, 4>(ctx, args, cache, grid, shm, dpart, st); break;
+      case 2: launch_bd<VT, 2>(ctx, args, cache, grid, shm, dpart, st); break;
+      default: launch_bd<VT, 1>(ctx, args, cache, grid, shm, dpart, st); break;
+    }
+    SBD_HIP_TRY(ctx, hipGetLastError());
+  } else {
 #define SBD_SIGMA_ARGS                                                                          \
   ctx->d_a, ctx->d_b, ctx->na, ctx->nb, ctx->norb, hii, T, W, accumulate, A.s_off, A.s_rec,     \
       A.d_off, A.d_rec, B.s_off, B.s_rec, B.d_off, B.ell64, B.ell32, B.ell_ket, B.ell_off,      \
@@ -1378,7 +1830,6 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
                          dim3(kSigBlock), shm, st, SBD_SIGMA_ARGS);                              \
   } while (0)
   // precomputed single-excitation elements, when both caches exist
-  const bool cache = sigma_cached(ctx);
   if (lds && rows == 8) {
     if constexpr (sizeof(VT) == 4) {
       if (cache) SBD_LAUNCH(true, true, 8);
@@ -1400,6 +1851,7 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
 #undef SBD_LAUNCH
 #undef SBD_SIGMA_ARGS
   SBD_HIP_TRY(ctx, hipGetLastError());
+  }
   if (skip_ab) {
     const SigmaArgs<VT> args = sigma_args<VT>(ctx, hii, T, W, 1);
     int C = 0;
@@ -1523,6 +1975,15 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
       (const void*)k_sigma_rows<float, true, false, 4, false>,
       (const void*)k_sigma_rows<float, true, true, 8, false>,
       (const void*)k_sigma_rows<float, true, false, 8, false>,
+#define SBD_BD_FNS(VT, R)                                                                         \
+  (const void*)k_sigma_bd<VT, R, true, true>, (const void*)k_sigma_bd<VT, R, true, false>,        \
+      (const void*)k_sigma_bd<VT, R, false, true>, (const void*)k_sigma_bd<VT, R, false, false>
+      SBD_BD_FNS(double, 1), SBD_BD_FNS(double, 2), SBD_BD_FNS(double, 4),
+      SBD_BD_FNS(float, 1), SBD_BD_FNS(float, 2), SBD_BD_FNS(float, 4), SBD_BD_FNS(float, 8),
+#undef SBD_BD_FNS
+      (const void*)k_sigma_bd_batch<double, 1>,
+      (const void*)k_sigma_bd_batch<double, 2>,
+      (const void*)k_sigma_bd_batch<double, 4>,
       (const void*)k_sigma_batch<double, 1>,
       (const void*)k_sigma_batch<double, 2>,
       (const void*)k_sigma_batch<double, 4>,

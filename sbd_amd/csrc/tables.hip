@@ -260,14 +260,23 @@ static int build_spin(sbd_ctx* ctx, const uint64_t* d_s, uint32_t n, int spin, S
     for (uint32_t g0 = 0; g0 < n; g0 += kColBlock) {
       const uint32_t g1 = std::min<uint64_t>(n, (uint64_t)g0 + kColBlock);
       for (uint32_t i = g0; i < g1; i++) col[i] = i;
-      std::stable_sort(col.begin() + g0, col.begin() + g1,
-                       [&](uint32_t x, uint32_t y) { return hd[x] > hd[y]; });
+      // strings with single excitations first (the sigma waits for their
+      // records only in the wavefronts that hold them), each group in
+      // descending order of its doubles count
+      std::stable_sort(col.begin() + g0, col.begin() + g1, [&](uint32_t x, uint32_t y) {
+        const bool sx = hs[x] != 0, sy = hs[y] != 0;
+        return sx != sy ? sx : hd[x] > hd[y];
+      });
     }
     std::vector<EllPos> pos(n);
-    for (uint32_t i = 0; i < n; i++) pos[i] = EllPos{col[i], hd[col[i]]};
+    std::vector<uint32_t> soff(n + 1, 0);
+    for (uint32_t i = 0; i < n; i++) soff[i + 1] = soff[i] + hs[i];
+    for (uint32_t i = 0; i < n; i++)
+      pos[i] = EllPos{col[i], hd[col[i]], soff[col[i]], soff[col[i] + 1]};
     SBD_HIP_TRY(ctx, hipMalloc(&T.ell_pos, sizeof(EllPos) * n));
     SBD_HIP_TRY(ctx,
                 hipMemcpy(T.ell_pos, pos.data(), sizeof(EllPos) * n, hipMemcpyHostToDevice));
+
     T.n_slices = (n + kEllSlice - 1) / kEllSlice;
     std::vector<uint32_t> eoff(T.n_slices + 1, 0);
     for (uint32_t s = 0; s < T.n_slices; s++) {

@@ -1,5 +1,6 @@
 // Wavefunction files: SaveWavefunction / LoadWavefunction
-// (chemistry/tpb/restart.h:20-65, 414-657) at one rank.
+// (chemistry/tpb/restart.h:20-65, 414-657) and SaveMatrixFormWF
+// (restart.h:663-780) at one rank.
 //
 // File layout (one file per rank; at one rank the name is the prefix followed
 // by "000000", to_padded_string(rank, 6), restart.h:14-18, 52):
@@ -12,6 +13,7 @@
 // in the current basis at their current position (others are dropped), then
 // normalises, as LoadWavefunction does.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -47,9 +49,64 @@ bool pack_words(const size_t* words, size_t bit_length, size_t wps, int norb, ui
   return true;
 }
 
+// get_extension (framework/bit_manipulation.h:1145-1166): the text after the
+// last '.', lower-cased, with NUL and space characters dropped past its first
+// character; empty when the name has no '.'.
+std::string file_extension(const std::string& path) {
+  const size_t dot = path.rfind('.');
+  if (dot == std::string::npos) return "";
+  std::string ext = path.substr(dot + 1);
+  for (char& ch : ext) ch = (char)std::tolower((unsigned char)ch);
+  std::string out;
+  for (size_t i = 0; i < ext.size(); i++)
+    if (i == 0 || (ext[i] != 0 && ext[i] != ' ')) out += ext[i];
+  return out;
+}
+
+// makestring (framework/bit_manipulation.h:196-211) of a one-word string:
+// orbitals norb - 1 down to 0 as '1' / '0'.
+std::string bit_text(uint64_t s, int norb) {
+  std::string t((size_t)norb, '0');
+  for (int p = 0; p < norb; p++)
+    if ((s >> p) & 1ULL) t[(size_t)(norb - 1 - p)] = '1';
+  return t;
+}
+
 }  // namespace
 
 extern "C" {
+
+int sbd_save_matrix_form_wf(const char* filename, int norb, const uint64_t* a, size_t na,
+                            const uint64_t* b, size_t nb, const double* W) {
+  if (!filename || norb < 1 || norb > 64 || (na && !a) || (nb && !b) || (na * nb && !W))
+    return SBD_ERR_ARG;
+  const std::string ext = file_extension(filename);
+  if (ext == "bin") {
+    // raw coefficients, W[ia * nb + ib] in index order
+    FILE* f = std::fopen(filename, "wb");
+    if (!f) return SBD_ERR_IO;
+    bool ok = std::fwrite(W, sizeof(double), na * nb, f) == na * nb;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? SBD_OK : SBD_ERR_IO;
+  }
+  if (ext != "dat" && ext != "txt") return SBD_OK;  // other extensions write nothing
+  // one line per determinant: "<value> # <ia>: <alpha string> <ib>: <beta
+  // string>", the value as an ostream with its default format writes a
+  // double (six significant digits, %g)
+  FILE* f = std::fopen(filename, "w");
+  if (!f) return SBD_ERR_IO;
+  bool ok = true;
+  std::vector<std::string> bt(nb);
+  for (size_t ib = 0; ib < nb; ib++) bt[ib] = bit_text(b[ib], norb);
+  for (size_t ia = 0; ia < na && ok; ia++) {
+    const std::string at = bit_text(a[ia], norb);
+    for (size_t ib = 0; ib < nb && ok; ib++)
+      ok = std::fprintf(f, "%g # %zu: %s %zu: %s\n", W[ia * nb + ib], ia, at.c_str(), ib,
+                        bt[ib].c_str()) > 0;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? SBD_OK : SBD_ERR_IO;
+}
 
 int sbd_save_wavefunction(sbd_ctx* ctx, const char* prefix, size_t bit_length, const double* W) {
   if (!ctx || !prefix || !W || bit_length == 0 || bit_length > 64) return SBD_ERR_ARG;

@@ -141,7 +141,7 @@ def test_tables_hdiag_sigma_bitexact(torch, name, factory):
     # the row kernel (with the dense alpha x beta pass where it applies), the
     # per-element kernel and the row kernel with every term inline give the
     # same bits, FP64 (overwrite and accumulate) and FP32
-    for kernel in ("rows", "elements", "rows_inline"):
+    for kernel in ("rows", "elements", "rows_inline", "split"):
         m.set_sigma_kernel(kernel)
         assert np.array_equal(m.sigma(hii, Wk).cpu().numpy(), s_ref), kernel
         Wacc2 = torch.tensor(W0, device="cuda")
