@@ -37,6 +37,11 @@
  *                        414-657                         sbd_save_wavefunction,
  *                                                        sbd_load_wavefunction
  *   diag with loadname   chemistry/tpb/sbdiag.h:339-347  sbd_diag_from
+ *   BasisInitVectorFromDeterminants chemistry/tpb/davidson.h:89-127 (diag with
+ *                        initial_*determinant_bitstring, sbdiag.h:303-335)
+ *                                                        sbd_diag_from_det
+ *   sbd::SaveMatrixFormWF chemistry/tpb/restart.h:663-780 (dump_matrix_form_wf,
+ *                        sbdiag.h:808-813)               sbd_save_matrix_form_wf
  *   sbd::SinglesExtendHalfdets chemistry/tpb/extend.h:14-185 (amplitude form)
  *                                                        sbd_singles_extend
  *                        chemistry/tpb/extend.h:186-324 (string form)

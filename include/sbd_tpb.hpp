@@ -7,12 +7,16 @@
  *   reference (include/sbd/...)                      here (namespace sbd_amd)
  *   FCIDump, LoadFCIDump   framework/fcidump.h:20-95  FCIDump, LoadFCIDump
  *   from_string            framework/bit_manipulation.h:219-235  from_string
+ *   from_string_checked    framework/determinant_initialization.h:18-34
+ *                                                     from_string_checked
  *   sort_bitarray          framework/bit_manipulation.h:421-449  sort_bitarray
  *   LoadAlphaDets          chemistry/basic/makedeterminants.h:167-191
  *                                                     LoadAlphaDets
  *   det_vector<size_t, det_kind::half>                det_vector (rows of words)
  *   tpb::SBD               chemistry/tpb/sbdiag.h:24-54  tpb::SBD
  *   tpb::diag (two forms)  chemistry/tpb/sbdiag.h:151-163, 846-857  tpb::diag
+ *                          (with loadname / savename, the explicit initial
+ *                          determinant and dump_matrix_form_wf)
  *   MultTPBThrust (makeQChamDiagTerms, run, correlation)
  *                          chemistry/tpb/mult_thrust.h:61-96  tpb::MultTPB
  *   Davidson(hii, W, mult, max_it, nb, eps, max_time)
@@ -33,6 +37,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -94,6 +99,20 @@ inline std::vector<size_t> from_string(const std::string& s, size_t bit_length, 
   for (size_t i = 0; i < L; i++)
     if (s[L - 1 - i] == '1') r[i / bit_length] |= (size_t)1 << (i % bit_length);
   return r;
+}
+
+// from_string_checked (framework/determinant_initialization.h:18-34): a bit
+// string of exactly total_bits characters, each '0' or '1'.
+inline std::vector<size_t> from_string_checked(const std::string& bitstring, size_t bit_length,
+                                               size_t total_bits) {
+  if (bit_length == 0 || bit_length > 8 * sizeof(size_t))
+    throw std::invalid_argument("invalid determinant bit length");
+  if (bitstring.size() != total_bits)
+    throw std::invalid_argument("determinant bitstring length does not match the basis");
+  for (const char v : bitstring)
+    if (v != '0' && v != '1')
+      throw std::invalid_argument("determinant bitstring must contain only 0 and 1");
+  return from_string(bitstring, bit_length, total_bits);
 }
 
 // Ascending by the last word first (the bit string's value), duplicates removed.
@@ -201,8 +220,6 @@ inline sbd_options options_of(const SBD& s) {
   if (s.task_comm_size != 1 || s.adet_comm_size != 1 || s.bdet_comm_size != 1 ||
       s.h_comm_size != 1)
     throw std::invalid_argument("sbd_amd: one rank per context (communicator sizes must be 1)");
-  if (!s.initial_adeterminant_bitstring.empty() || !s.initial_bdeterminant_bitstring.empty())
-    throw std::invalid_argument("sbd_amd: initial determinant bit strings are not supported");
   sbd_options o;
   sbd_options_default(&o);
   o.method = s.method;
@@ -332,13 +349,43 @@ inline void diag(const SBD& sbd_data, const FCIDump& fcidump, const det_vector& 
                  std::vector<std::vector<double>>& two_p_rdm) {
   const sbd_options o = options_of(sbd_data);
   const size_t L = norb_of(fcidump), bl = sbd_data.bit_length;
+  // explicit initial determinant (sbdiag.h:303-335): checked before the solve
+  // as BasisInitVectorFromDeterminants checks it (davidson.h:97-110)
+  const bool has_a = !sbd_data.initial_adeterminant_bitstring.empty();
+  const bool has_b = !sbd_data.initial_bdeterminant_bitstring.empty();
+  uint64_t init_a = 0, init_b = 0;
+  if (loadname.empty() && has_a) {
+    const auto ra = from_string_checked(sbd_data.initial_adeterminant_bitstring, bl, L);
+    const auto rb = from_string_checked(
+        has_b ? sbd_data.initial_bdeterminant_bitstring : sbd_data.initial_adeterminant_bitstring,
+        bl, L);
+    auto unique_in = [](const det_vector& basis, const std::vector<size_t>& want,
+                        const char* name) {
+      const auto it = std::find(basis.begin(), basis.end(), want);
+      if (it == basis.end())
+        throw std::invalid_argument(std::string("initial ") + name +
+                                    " is not present in the basis");
+      if (std::find(std::next(it), basis.end(), want) != basis.end())
+        throw std::invalid_argument(std::string("initial ") + name +
+                                    " occurs more than once in the basis");
+    };
+    unique_in(adet, ra, "alpha determinant");
+    unique_in(bdet, rb, "beta determinant");
+    init_a = pack({ra}, bl, L)[0];
+    init_b = pack({rb}, bl, L)[0];
+  } else if (loadname.empty() && has_b) {
+    throw std::invalid_argument("an initial beta determinant requires an initial alpha determinant");
+  }
   MultTPB m(adet, bdet, bl, L, fcidump);
   sbd_ctx* c = m.handle();
   const size_t N = m.size();
   std::vector<double> W(N);
   sbd_report rep;
   if (loadname.empty()) {
-    check(sbd_diag(c, &o, &energy, W.data(), nullptr, 0, &rep), c);
+    if (has_a)
+      check(sbd_diag_from_det(c, &o, init_a, init_b, &energy, W.data(), nullptr, 0, &rep), c);
+    else
+      check(sbd_diag(c, &o, &energy, W.data(), nullptr, 0, &rep), c);
   } else {
     std::vector<double> W0(N);
     uint64_t placed = 0;
@@ -347,6 +394,14 @@ inline void diag(const SBD& sbd_data, const FCIDump& fcidump, const det_vector& 
     check(sbd_diag_from(c, &o, W0.data(), &energy, W.data(), nullptr, 0, &rep), c);
   }
   if (!savename.empty()) check(sbd_save_wavefunction(c, savename.c_str(), bl, W.data()), c);
+  if (!sbd_data.dump_matrix_form_wf.empty()) {
+    // SaveMatrixFormWF (sbdiag.h:808-813)
+    const std::vector<uint64_t> pa = pack(adet, bl, L), pb = pack(bdet, bl, L);
+    const int st = sbd_save_matrix_form_wf(sbd_data.dump_matrix_form_wf.c_str(), (int)L,
+                                           pa.data(), pa.size(), pb.data(), pb.size(), W.data());
+    if (st != SBD_OK)
+      throw std::runtime_error("sbd_amd: cannot write " + sbd_data.dump_matrix_form_wf);
+  }
   detail::DevVec dW(N);
   check_hip(hipMemcpy(dW.p, W.data(), 8 * N, hipMemcpyHostToDevice), "hipMemcpy");
   density.assign(2 * L, 0.0);

@@ -191,7 +191,10 @@ class SBD:
     max_time: float = 86400.0
     init: int = 0
     seed: int = 1729
+    initial_adeterminant_bitstring: str = ""  # explicit initial determinant (alpha string)
+    initial_bdeterminant_bitstring: str = ""  # its beta string (default: the alpha one)
     bit_length: int = 20
+    dump_matrix_form_wf: str = ""  # SaveMatrixFormWF file (.bin / .dat / .txt)
     do_rdm: int = 0              # 0: occupation density; else also the full 1- and 2-RDMs
     carryover_type: int = 0      # 0 none, 1 dominant strings, 2 + singles, 3 singles of W^2 > threshold
     ratio: float = 0.0
@@ -252,6 +255,32 @@ def from_string(s: str, L: int) -> int:
         if s[L - 1 - i] == "1":
             v |= 1 << i
     return v
+
+
+def from_string_checked(bitstring: str, bit_length: int, total_bits: int) -> int:
+    """framework/determinant_initialization.h:18-34: a bit string of exactly
+    total_bits characters '0'/'1' (ValueError otherwise), as one integer
+    (orbital p -> bit p)."""
+    if bit_length == 0 or bit_length > 64:
+        raise ValueError("invalid determinant bit length")
+    if len(bitstring) != total_bits:
+        raise ValueError("determinant bitstring length does not match the basis")
+    if any(ch not in "01" for ch in bitstring):
+        raise ValueError("determinant bitstring must contain only 0 and 1")
+    return from_string(bitstring, total_bits)
+
+
+def save_matrix_form_wf(filename: str, norb: int, a, b, W):
+    """SaveMatrixFormWF (tpb/restart.h:663-780) at one rank: by extension,
+    "bin" writes W's doubles, "dat"/"txt" one line per determinant
+    ("<W> # <ia>: <alpha bits> <ib>: <beta bits>"), others nothing."""
+    a, b = _u64(a), _u64(b)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    if W.size != len(a) * len(b):
+        raise SbdError("W must have len(a) * len(b) elements")
+    _check(lib().sbd_save_matrix_form_wf(str(filename).encode(), int(norb), _ptr(a, C.c_uint64),
+                                         len(a), _ptr(b, C.c_uint64), len(b),
+                                         _ptr(W, C.c_double)))
 
 
 def LoadAlphaDets(path: str, L: int) -> np.ndarray:
@@ -469,18 +498,25 @@ class MultTPB:
                                            _ptr(W, C.c_double), C.byref(placed)), self._ctx)
         return W, placed.value
 
-    def diag(self, sbd: SBD, want_vector=False, history=False, W_init=None):
+    def diag(self, sbd: SBD, want_vector=False, history=False, W_init=None, start_det=None):
         """sbd::tpb::diag: method 0/1 Davidson, 2/3 Lanczos.  Returns (energy,
         W or None, report[, history]); Davidson history rows are (E0, |R|,
         seconds since solver start, vector precision 32/64), Lanczos rows
-        (E0, beta, seconds, 64)."""
+        (E0, beta, seconds, 64).  W_init: start vector (host, N); start_det:
+        (alpha string, beta string) whose unit vector starts the solve
+        (BasisInitVectorFromDeterminants)."""
         o = sbd.options()
         E = C.c_double(0.0)
         rep = Report()
         W = np.zeros(self.N) if want_vector else None
         cap = sbd.max_it * sbd.max_nb * (2 if sbd.precision == 1 else 1)
         hist = np.zeros(4 * cap)
-        if W_init is not None:
+        if start_det is not None:
+            _check(lib().sbd_diag_from_det(self._ctx, C.byref(o), C.c_uint64(int(start_det[0])),
+                                           C.c_uint64(int(start_det[1])), C.byref(E),
+                                           _ptr(W, C.c_double) if want_vector else None,
+                                           _ptr(hist, C.c_double), cap, C.byref(rep)), self._ctx)
+        elif W_init is not None:
             W0 = np.ascontiguousarray(W_init, dtype=np.float64)
             if W0.size != self.N:
                 raise SbdError("W_init must have N elements")
@@ -599,6 +635,25 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
     import torch
     if sbd_data.carryover_type not in (0, 1, 2, 3):
         raise SbdError("carryover_type must be 0..3")
+    start = None
+    if not loadname:
+        # explicit initial determinant (sbdiag.h:303-335), checked as
+        # BasisInitVectorFromDeterminants checks it (davidson.h:97-110)
+        sa, sb = sbd_data.initial_adeterminant_bitstring, sbd_data.initial_bdeterminant_bitstring
+        if sb and not sa:
+            raise ValueError("an initial beta determinant requires an initial alpha determinant")
+        if sa:
+            L = fcidump.norb
+            want = (from_string_checked(sa, sbd_data.bit_length, L),
+                    from_string_checked(sb or sa, sbd_data.bit_length, L))
+            for name, basis, x in (("alpha", adet, want[0]), ("beta", bdet, want[1])):
+                hits = int(np.count_nonzero(np.asarray(basis, dtype=np.uint64) == np.uint64(x)))
+                if hits == 0:
+                    raise ValueError(f"initial {name} determinant is not present in the basis")
+                if hits > 1:
+                    raise ValueError(f"initial {name} determinant occurs more than once in the "
+                                     "basis")
+            start = want
     m = MultTPB(fcidump.norb, adet, bdet, fcidump=fcidump)
     try:
         W0 = None
@@ -607,9 +662,12 @@ def diag(sbd_data: SBD, fcidump: FCIDump, adet, bdet, loadname="", savename=""):
             W0, placed = m.load_wavefunction(loadname, sbd_data.bit_length)
             if placed == 0:
                 raise SbdError(f"{loadname}: no saved coefficient lies in this basis")
-        E, W, _ = m.diag(sbd_data, want_vector=True, W_init=W0)
+        E, W, _ = m.diag(sbd_data, want_vector=True, W_init=W0, start_det=start)
         if savename:
             m.save_wavefunction(savename, W, sbd_data.bit_length)
+        if sbd_data.dump_matrix_form_wf:
+            # SaveMatrixFormWF (sbdiag.h:808-813)
+            save_matrix_form_wf(sbd_data.dump_matrix_form_wf, m.norb, adet, bdet, W)
         Wd = torch.tensor(W, device="cuda")
         one = two = np.zeros((0, 0))
         if sbd_data.do_rdm == 0:

@@ -6,6 +6,7 @@
 //   test_tpb_host --gpu FCIDUMP ALPHA  also the solves on H2O 1e-3 (needs a GPU)
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -158,15 +159,131 @@ void test_diag_h2o(const std::string& fcidump, const std::string& alpha) {
   TEST_ASSERT(std::fabs(e - E) < 1e-8 && dmax < 1e-12);
 }
 
+// from_string_checked: the reference's known answers
+// (tests/functionality/test_initial_determinant.cc:16-27) and the bit-length
+// checks of determinant_initialization.h:21-23
+void test_from_string_checked() {
+  const auto r = from_string_checked("10000001", 4, 8);
+  TEST_ASSERT(r.size() == 2 && r[0] == 1 && r[1] == 8);
+  const auto v = from_string_checked("00101", 20, 5);
+  TEST_ASSERT(v.size() == 1 && v[0] == 0b101);
+  auto throws = [](const std::string& s, size_t bl, size_t n) {
+    try {
+      from_string_checked(s, bl, n);
+    } catch (const std::invalid_argument&) {
+      return true;
+    }
+    return false;
+  };
+  TEST_ASSERT(throws("1000000", 4, 8));   // one character short
+  TEST_ASSERT(throws("1000000x", 4, 8));  // not a bit
+  TEST_ASSERT(throws("0101", 20, 5));    // too short
+  TEST_ASSERT(throws("001011", 20, 5));  // too long
+  TEST_ASSERT(throws("00a01", 20, 5));   // not a bit
+  TEST_ASSERT(throws("00101", 0, 5));    // bit length 0
+  TEST_ASSERT(throws("00101", 65, 5));   // bit length above the word
+}
+
+// SaveMatrixFormWF (restart.h:663-780) layouts, host only
+void test_save_matrix_form_wf() {
+  const uint64_t a[2] = {0b011, 0b101}, b[2] = {0b011, 0b110};
+  const double W[4] = {0.5, -1.25e-7, 1.0 / 3.0, 0.0};
+  TEST_ASSERT(sbd_save_matrix_form_wf("mf.bin", 3, a, 2, b, 2, W) == SBD_OK);
+  FILE* f = std::fopen("mf.bin", "rb");
+  double got[5] = {0, 0, 0, 0, 0};
+  const size_t n = f ? std::fread(got, sizeof(double), 5, f) : 0;
+  if (f) std::fclose(f);
+  TEST_ASSERT(n == 4 && got[0] == W[0] && got[1] == W[1] && got[2] == W[2] && got[3] == W[3]);
+  TEST_ASSERT(sbd_save_matrix_form_wf("mf.TXT", 3, a, 2, b, 2, W) == SBD_OK);
+  std::ifstream t("mf.TXT");
+  std::vector<std::string> lines;
+  for (std::string l; std::getline(t, l);) lines.push_back(l);
+  TEST_ASSERT(lines.size() == 4);
+  TEST_ASSERT(lines.size() == 4 && lines[0] == "0.5 # 0: 011 0: 011" &&
+              lines[1] == "-1.25e-07 # 0: 011 1: 110" && lines[2] == "0.333333 # 1: 101 0: 011" &&
+              lines[3] == "0 # 1: 101 1: 110");
+  // any other extension writes nothing
+  std::remove("mf.wf");
+  TEST_ASSERT(sbd_save_matrix_form_wf("mf.wf", 3, a, 2, b, 2, W) == SBD_OK);
+  TEST_ASSERT(std::fopen("mf.wf", "rb") == nullptr);
+  std::remove("mf.bin");
+  std::remove("mf.TXT");
+}
+
+// explicit initial determinant and matrix-form dump through tpb::diag
+void test_initial_determinant_h2o(const std::string& fcidump, const std::string& alpha) {
+  const FCIDump fd = LoadFCIDump(fcidump);
+  det_vector a;
+  tpb::SBD s;
+  s.max_it = 20;
+  s.max_nb = 10;
+  s.eps = 1e-8;
+  LoadAlphaDets(alpha, a, s.bit_length, 24);
+  double E0 = 0.0, E1 = 0.0;
+  std::vector<double> density;
+  det_vector co_a, co_b;
+  std::vector<std::vector<double>> one, two;
+  tpb::diag(s, fd, a, a, "", "", E0, density, co_a, co_b, one, two);
+  // the Hartree-Fock string is the first one: the same start as init 0
+  s.initial_adeterminant_bitstring = "000000000000000000011111";
+  s.dump_matrix_form_wf = "h2o_mf.bin";
+  tpb::diag(s, fd, a, a, "", "", E1, density, co_a, co_b, one, two);
+  std::printf("  explicit initial determinant: E = %.10f (init 0: %.10f)\n", E1, E0);
+  TEST_ASSERT(E1 == E0);
+  FILE* f = std::fopen("h2o_mf.bin", "rb");
+  TEST_ASSERT(f != nullptr);
+  if (f) {
+    std::fseek(f, 0, SEEK_END);
+    const long bytes = std::ftell(f);
+    std::fclose(f);
+    TEST_ASSERT((size_t)bytes == 8 * a.size() * a.size());
+  }
+  std::remove("h2o_mf.bin");
+  s.dump_matrix_form_wf.clear();
+  auto throws_msg = [&](const std::string& sa, const std::string& sb, const std::string& want) {
+    tpb::SBD t = s;
+    t.initial_adeterminant_bitstring = sa;
+    t.initial_bdeterminant_bitstring = sb;
+    double E = 0.0;
+    try {
+      tpb::diag(t, fd, a, a, "", "", E, density, co_a, co_b, one, two);
+    } catch (const std::invalid_argument& ex) {
+      return std::string(ex.what()) == want;
+    }
+    return false;
+  };
+  TEST_ASSERT(throws_msg("", "000000000000000000011111",
+                         "an initial beta determinant requires an initial alpha determinant"));
+  TEST_ASSERT(throws_msg("111110000000000000000000", "",
+                         "initial alpha determinant is not present in the basis"));
+  TEST_ASSERT(throws_msg("000000000000000000011111", "111110000000000000000000",
+                         "initial beta determinant is not present in the basis"));
+  det_vector dup = a;
+  dup.push_back(a[0]);
+  tpb::SBD t = s;
+  t.initial_adeterminant_bitstring = "000000000000000000011111";
+  bool threw = false;
+  try {
+    double E = 0.0;
+    tpb::diag(t, fd, dup, a, "", "", E, density, co_a, co_b, one, two);
+  } catch (const std::invalid_argument& ex) {
+    threw = std::string(ex.what()) == "initial alpha determinant occurs more than once in the basis";
+  }
+  TEST_ASSERT(threw);
+}
+
 int main(int argc, char** argv) {
   test_from_string_kat();
   test_sort_bitarray();
   test_pack_roundtrip();
   test_options_defaults();
+  test_from_string_checked();
+  test_save_matrix_form_wf();
   if (argc >= 4) test_load_inputs(argv[2], argv[3]);
   if (argc >= 4 && std::string(argv[1]) == "--gpu") {
     try {
       test_diag_h2o(argv[2], argv[3]);
+      test_initial_determinant_h2o(argv[2], argv[3]);
     } catch (const std::exception& ex) {
       std::printf("FAIL exception: %s\n", ex.what());
       g_fail++;

@@ -126,3 +126,55 @@ def test_warm_start_h2o(torch, tmp_path):
     assert rw["steps"] < rc["steps"] // 3, (rw["steps"], rc["steps"])
     r2 = sbd_amd.diag(opt, fd, a, a, loadname=prefix)
     assert abs(r2.energy - E_cold) < 1e-9
+
+
+def test_initial_determinant_and_matrix_form_h2o(torch, tmp_path):
+    """diag from an explicit initial determinant (initial_*determinant_
+    bitstring, sbdiag.h:303-335; BasisInitVectorFromDeterminants,
+    davidson.h:89-127): from the Hartree-Fock string (index 0) it is init 0,
+    bit for bit; from another determinant it equals the solve started from
+    that unit vector.  The reference's errors for a missing string, a beta
+    string without an alpha one and a string present twice.  SaveMatrixFormWF
+    (dump_matrix_form_wf): the .bin file holds the solve's W."""
+    import sbd_amd
+    from oracle import oracle as O
+    say("initial determinant h2o")
+    hdr, vals, idx = O.load_fcidump(os.path.join(GOLDEN, "h2o", "fcidump.txt.gz"))
+    L = int(hdr["NORB"])
+    a = O.load_alpha_dets(os.path.join(GOLDEN, "h2o", "h2o-1em3-alpha.txt.gz"), L)
+    fd = sbd_amd.FCIDump({"NORB": str(L)}, vals, idx)
+    bits = lambda s: format(int(s), f"0{L}b")
+    opt = sbd_amd.SBD(max_it=20, max_nb=10, eps=1e-8)
+    r0 = sbd_amd.diag(opt, fd, a, a)
+    hf = sbd_amd.SBD(max_it=20, max_nb=10, eps=1e-8, initial_adeterminant_bitstring=bits(a[0]),
+                     dump_matrix_form_wf=str(tmp_path / "h2o_wf.bin"))
+    r1 = sbd_amd.diag(hf, fd, a, a)
+    assert r1.energy == r0.energy
+    Wf = np.fromfile(str(tmp_path / "h2o_wf.bin"), dtype="<f8")
+    assert Wf.size == len(a) ** 2 and np.array_equal(Wf, r1.W)
+    # another determinant: alpha string 3, beta string 5
+    m = sbd_amd.MultTPB(L, a, a, fcidump=fd)
+    e = np.zeros(m.N)
+    e[3 * m.nb + 5] = 1.0
+    E_unit, _, _ = m.diag(opt, W_init=e)
+    E_det, _, _ = m.diag(opt, start_det=(a[3], a[5]))
+    m.close()
+    # (this start may lie in another symmetry sector than the ground state,
+    # where the solve stays: only the variational bound holds)
+    assert E_det == E_unit and E_det >= r0.energy - 1e-9
+    od = sbd_amd.SBD(max_it=20, max_nb=10, eps=1e-8, initial_adeterminant_bitstring=bits(a[3]),
+                     initial_bdeterminant_bitstring=bits(a[5]))
+    assert sbd_amd.diag(od, fd, a, a).energy == E_det
+    say("initial determinant errors")
+    missing = bits(sum(1 << p for p in range(L - 5, L)))
+    for sa, sb, msg in (("", bits(a[0]), "requires an initial alpha"),
+                        (missing, "", "initial alpha determinant is not present"),
+                        (bits(a[0]), missing, "initial beta determinant is not present"),
+                        (bits(a[0])[1:], "", "length does not match")):
+        bad = sbd_amd.SBD(max_it=2, initial_adeterminant_bitstring=sa,
+                          initial_bdeterminant_bitstring=sb)
+        with pytest.raises(ValueError, match=msg):
+            sbd_amd.diag(bad, fd, a, a)
+    dup = np.concatenate([a, a[:1]])
+    with pytest.raises(ValueError, match="occurs more than once"):
+        sbd_amd.diag(sbd_amd.SBD(initial_adeterminant_bitstring=bits(a[0])), fd, dup, a)

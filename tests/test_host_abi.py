@@ -174,3 +174,35 @@ def test_binary_strings_kats(L, tmp_path):
         if Lo % 8 == 0:   # writer and reader agree when there is no padding
             sbd_amd.encode_strings_bin(p, s, Lo)
             assert np.array_equal(sbd_amd.DecodeAlphaDets(p, Lo), s)
+
+
+def test_from_string_checked_kats():
+    """determinant_initialization.h:18-34 with the reference's known answers
+    (test_initial_determinant.cc:16-27)."""
+    import sbd_amd
+    assert sbd_amd.from_string_checked("10000001", 4, 8) == 0b10000001
+    for s, bl, n in (("1000000", 4, 8), ("1000000x", 4, 8), ("00101", 0, 5)):
+        with pytest.raises(ValueError):
+            sbd_amd.from_string_checked(s, bl, n)
+
+
+def test_save_matrix_form_wf_layouts(L, tmp_path):
+    """SaveMatrixFormWF (restart.h:663-780) at one rank: "bin" is W's doubles
+    as they lie; "dat"/"txt" (any case; get_extension lower-cases and drops
+    spaces) is one line per determinant, value as a default ostream prints a
+    double (six significant digits), then "# ia: <alpha bits> ib: <beta bits>"
+    with orbital norb - 1 first (makestring); other extensions write nothing."""
+    import sbd_amd
+    a = np.array([0b011, 0b101], dtype=np.uint64)
+    b = np.array([0b011, 0b110], dtype=np.uint64)
+    W = np.array([0.5, -1.25e-7, 1.0 / 3.0, 0.0])
+    sbd_amd.save_matrix_form_wf(tmp_path / "w.bin", 3, a, b, W)
+    assert (tmp_path / "w.bin").read_bytes() == W.astype("<f8").tobytes()
+    for name in ("w.txt", "w.DAT", "w.tx t"):
+        sbd_amd.save_matrix_form_wf(tmp_path / name, 3, a, b, W)
+        assert (tmp_path / name).read_text().splitlines() == [
+            "0.5 # 0: 011 0: 011", "-1.25e-07 # 0: 011 1: 110",
+            "0.333333 # 1: 101 0: 011", "0 # 1: 101 1: 110"]
+    sbd_amd.save_matrix_form_wf(tmp_path / "w.wf", 3, a, b, W)
+    sbd_amd.save_matrix_form_wf(tmp_path / "noext", 3, a, b, W)
+    assert not (tmp_path / "w.wf").exists() and not (tmp_path / "noext").exists()
