@@ -152,15 +152,17 @@ sbd_ctx* sbd_ctx_create(int norb, double I0, const double* h1, const double* eri
 void sbd_ctx_destroy(sbd_ctx* ctx);
 
 /* Sigma kernel choice (no counterpart in the reference: a tuning knob of this
- * library).  0 (default): by sparsity -- bases with fewer than 2 off-diagonal
- * pairs per alpha string plus per beta string use the per-element kernel,
- * others the row kernel, whose alpha-single x beta-single terms go to a pass
- * of their own in bases with at least 4 of them per determinant; 1: row
- * kernel (with that pass where it applies); 2: per-element kernel; 3: row
- * kernel with every term inline; 4: split form -- a streaming kernel for the
+ * library).  0 (default): bases with fewer than 2 off-diagonal pairs per
+ * alpha string plus per beta string, or whose bra rows do not fit LDS, use the
+ * per-element kernel; others the split form (a streaming kernel for the
  * diagonal and alpha terms over column slabs, then a row kernel for the beta
- * and alpha-single x beta-single terms (bases whose bra rows fit LDS).  All
- * give the same bits, and the serving engine's batches follow the same rule. */
+ * and alpha-single x beta-single terms) when two or more rows fit LDS
+ * together, else the row kernel; in bases with at least 4 alpha-single x
+ * beta-single terms per determinant those terms go to a pass of their own.
+ * 1: row kernel (with that pass where it applies); 2: per-element kernel;
+ * 3: row kernel with every term inline; 4: split form (where one row fits
+ * LDS).  All give the same bits, and the serving engine's batches follow the
+ * same rule. */
 int sbd_set_sigma_kernel(sbd_ctx* ctx, int kernel);
 /* Slab width (columns: 256, 512 or 1024) of the split form's streaming kernel
  * (no counterpart in the reference: a tuning knob; results do not depend on it). */
