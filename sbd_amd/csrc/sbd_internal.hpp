@@ -237,7 +237,8 @@ struct sbd_ctx {
   // the null stream unless a caller such as the serving engine sets its own
   hipStream_t stream = nullptr;
   int sigma_kernel = 0;  // 0 = by sparsity, 1 = row kernel, 2 = per-element, 3 = rows, alpha x beta inline, 4 = split
-  int stream_cols = 512;  // slab width of the split sigma's streaming kernel (256, 512, 1024)
+  int stream_cols = 512;  // slab width of the split sigma's streaming kernel (256, 512, 1024;
+                          // set from nb at creation, sbd_set_sigma_slab)
   int ab_pass = -1;      // dense alpha x beta pass: -1 not yet decided, 0 no, 1 yes
   size_t ab_lds = 0;     // its LDS bytes per workgroup
   uint32_t* d_ab_sell = nullptr;  // its column-major packed beta single records
