@@ -14,6 +14,7 @@
  *                                                     LoadAlphaDets
  *   det_vector<size_t, det_kind::half>                det_vector (rows of words)
  *   tpb::SBD               chemistry/tpb/sbdiag.h:24-54  tpb::SBD
+ *   generate_sbd_data      chemistry/tpb/sbdiag.h:56-134  tpb::generate_sbd_data
  *   tpb::diag (two forms)  chemistry/tpb/sbdiag.h:151-163, 846-857  tpb::diag
  *                          (with loadname / savename, the explicit initial
  *                          determinant and dump_matrix_form_wf)
@@ -37,6 +38,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <iterator>
 #include <map>
 #include <memory>
@@ -215,6 +217,49 @@ struct SBD {
   int precision = 0;       // added: 1 = FP32 vectors and sigma, then an FP64 sweep
   double fp32_eps = 1e-2;  // added: residual at which FP32 hands over to FP64
 };
+
+// generate_sbd_data (sbdiag.h:56-134): the example programs' command-line
+// flags into an SBD, scanned as the reference scans them (each flag takes the
+// next argument, read with atoi / atof; unknown arguments are skipped).  The
+// reference's Thrust-only flags (--use_precalculated_dets,
+// --max_memory_gb_for_determinants, --thrust_collapse_loops) are accepted and
+// have no effect here.  A flag without its value throws std::invalid_argument
+// (the reference would read past argv).
+inline SBD generate_sbd_data(int argc, char* argv[]) {
+  SBD d;
+  for (int i = 0; i < argc; i++) {
+    auto val = [&]() -> const char* {
+      if (i + 1 >= argc) throw std::invalid_argument(std::string(argv[i]) + " needs a value");
+      return argv[++i];
+    };
+    auto is = [&](const char* a) { return std::string(argv[i]) == a; };
+    if (is("--adet_comm_size")) d.adet_comm_size = std::atoi(val());
+    if (is("--bdet_comm_size")) d.bdet_comm_size = std::atoi(val());
+    if (is("--task_comm_size")) d.task_comm_size = std::atoi(val());
+    if (is("--method")) d.method = std::atoi(val());
+    if (is("--init")) d.init = std::atoi(val());
+    if (is("--initial_adeterminant_bitstring") || is("--initial-adeterminant-bitstring"))
+      d.initial_adeterminant_bitstring = val();
+    if (is("--initial_bdeterminant_bitstring") || is("--initial-bdeterminant-bitstring"))
+      d.initial_bdeterminant_bitstring = val();
+    if (is("--seed")) d.seed = (size_t)std::atoi(val());
+    if (is("--iteration")) d.max_it = std::atoi(val());
+    if (is("--block")) d.max_nb = std::atoi(val());
+    if (is("--tolerance")) d.eps = std::atof(val());
+    if (is("--carryover_type")) d.carryover_type = std::atoi(val());
+    if (is("--carryover_ratio")) d.ratio = std::atof(val());
+    if (is("--carryover_threshold")) d.threshold = std::atof(val());
+    if (is("--max_time")) d.max_time = std::atof(val());
+    if (is("--shuffle")) d.do_shuffle = std::atoi(val());
+    if (is("--rdm")) d.do_rdm = std::atoi(val());
+    if (is("--bit_length")) d.bit_length = (size_t)std::atoi(val());
+    if (is("--dump_matrix_form_wf")) d.dump_matrix_form_wf = val();
+    if (is("--use_precalculated_dets") || is("--max_memory_gb_for_determinants") ||
+        is("--thrust_collapse_loops"))
+      (void)val();
+  }
+  return d;
+}
 
 inline sbd_options options_of(const SBD& s) {
   if (s.task_comm_size != 1 || s.adet_comm_size != 1 || s.bdet_comm_size != 1 ||

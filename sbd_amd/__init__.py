@@ -212,6 +212,68 @@ class SBD:
         return o
 
 
+def _c_number(text: str, integer: bool):
+    """The number C's atoi / atof reads at the start of text (0 if none)."""
+    import re
+    pat = r"\s*[+-]?\d+" if integer else r"\s*[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?"
+    m = re.match(pat, text)
+    if not m:
+        return 0 if integer else 0.0
+    return int(m.group(0)) if integer else float(m.group(0))
+
+
+def generate_sbd_data(argv) -> SBD:
+    """generate_sbd_data (chemistry/tpb/sbdiag.h:56-134): the example programs'
+    flags into an SBD, scanned as the reference scans them (the flags tested in
+    its order against the current argument, each flag taking the next argument,
+    read as C's atoi / atof would; unknown arguments are skipped).  The
+    reference's Thrust-only flags and --shuffle are accepted and ignored; the
+    communicator sizes must be 1.  A flag without its value raises ValueError."""
+    d = SBD()
+    comm = {}
+    # (flags, kind, field): the reference's order
+    table = [(("--adet_comm_size",), "comm", None), (("--bdet_comm_size",), "comm", None),
+             (("--task_comm_size",), "comm", None), (("--method",), "i", "method"),
+             (("--init",), "i", "init"),
+             (("--initial_adeterminant_bitstring", "--initial-adeterminant-bitstring"), "s",
+              "initial_adeterminant_bitstring"),
+             (("--initial_bdeterminant_bitstring", "--initial-bdeterminant-bitstring"), "s",
+              "initial_bdeterminant_bitstring"),
+             (("--seed",), "i", "seed"), (("--iteration",), "i", "max_it"),
+             (("--block",), "i", "max_nb"), (("--tolerance",), "f", "eps"),
+             (("--carryover_type",), "i", "carryover_type"),
+             (("--carryover_ratio",), "f", "ratio"), (("--carryover_threshold",), "f", "threshold"),
+             (("--max_time",), "f", "max_time"), (("--shuffle",), "skip", None),
+             (("--rdm",), "i", "do_rdm"), (("--bit_length",), "i", "bit_length"),
+             (("--dump_matrix_form_wf",), "s", "dump_matrix_form_wf"),
+             (("--use_precalculated_dets", "--max_memory_gb_for_determinants",
+               "--thrust_collapse_loops"), "skip", None)]
+    argv = list(argv)
+    i = 0
+    while i < len(argv):
+        for flags, kind, field in table:
+            if argv[i] not in flags:
+                continue
+            if i + 1 >= len(argv):
+                raise ValueError(f"{argv[i]} needs a value")
+            key = argv[i]
+            i += 1
+            v = argv[i]
+            if kind == "i":
+                setattr(d, field, _c_number(v, True))
+            elif kind == "f":
+                setattr(d, field, _c_number(v, False))
+            elif kind == "s":
+                setattr(d, field, v)
+            elif kind == "comm":
+                comm[key] = _c_number(v, True)
+        i += 1
+    for key, n in comm.items():
+        if n != 1:
+            raise SbdError(f"{key} {n}: one rank per context (communicator sizes must be 1)")
+    return d
+
+
 def _check(rc, ctx=None):
     if rc != 0:
         msg = lib().sbd_status_string(rc).decode()

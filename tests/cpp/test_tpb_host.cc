@@ -272,6 +272,26 @@ void test_initial_determinant_h2o(const std::string& fcidump, const std::string&
   TEST_ASSERT(threw);
 }
 
+// generate_sbd_data (sbdiag.h:56-134)
+void test_generate_sbd_data() {
+  const char* av[] = {"prog", "--iteration", "20", "--block", "8", "--tolerance", "1e-8",
+                      "--rdm", "1", "--initial-adeterminant-bitstring", "0011",
+                      "--dump_matrix_form_wf", "w.txt", "--thrust_collapse_loops", "1",
+                      "--bdet_comm_size", "2", "other"};
+  const int ac = (int)(sizeof(av) / sizeof(av[0]));
+  const tpb::SBD d = tpb::generate_sbd_data(ac, const_cast<char**>(av));
+  TEST_ASSERT(d.max_it == 20 && d.max_nb == 8 && d.eps == 1e-8 && d.do_rdm == 1);
+  TEST_ASSERT(d.initial_adeterminant_bitstring == "0011" && d.dump_matrix_form_wf == "w.txt");
+  TEST_ASSERT(d.bdet_comm_size == 2 && d.method == 0 && d.seed == 1729);
+  bool threw = false;
+  try {
+    tpb::options_of(d);  // communicator sizes other than 1 are refused at the solve
+  } catch (const std::invalid_argument&) {
+    threw = true;
+  }
+  TEST_ASSERT(threw);
+}
+
 int main(int argc, char** argv) {
   test_from_string_kat();
   test_sort_bitarray();
@@ -279,6 +299,7 @@ int main(int argc, char** argv) {
   test_options_defaults();
   test_from_string_checked();
   test_save_matrix_form_wf();
+  test_generate_sbd_data();
   if (argc >= 4) test_load_inputs(argv[2], argv[3]);
   if (argc >= 4 && std::string(argv[1]) == "--gpu") {
     try {

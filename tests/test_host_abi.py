@@ -206,3 +206,27 @@ def test_save_matrix_form_wf_layouts(L, tmp_path):
     sbd_amd.save_matrix_form_wf(tmp_path / "w.wf", 3, a, b, W)
     sbd_amd.save_matrix_form_wf(tmp_path / "noext", 3, a, b, W)
     assert not (tmp_path / "w.wf").exists() and not (tmp_path / "noext").exists()
+
+
+def test_generate_sbd_data_flags():
+    """generate_sbd_data (sbdiag.h:56-134): every flag of the reference into its
+    field, atoi / atof reading, both spellings of the initial-determinant
+    flags, Thrust-only flags ignored, defaults kept otherwise."""
+    import sbd_amd
+    argv = ["prog", "--method", "2", "--init", "1", "--seed", "12x", "--iteration", "20",
+            "--block", "8", "--tolerance", "1e-8", "--carryover_type", "3",
+            "--carryover_ratio", "0.25", "--carryover_threshold", ".5e-2", "--max_time", "60",
+            "--shuffle", "1", "--rdm", "1", "--bit_length", "64",
+            "--initial-adeterminant-bitstring", "0011", "--initial_bdeterminant_bitstring",
+            "0101", "--dump_matrix_form_wf", "w.txt", "--use_precalculated_dets", "1",
+            "--adet_comm_size", "1", "unknown"]
+    d = sbd_amd.generate_sbd_data(argv)
+    assert (d.method, d.init, d.seed, d.max_it, d.max_nb) == (2, 1, 12, 20, 8)
+    assert (d.eps, d.carryover_type, d.ratio, d.threshold, d.max_time) == (1e-8, 3, 0.25, 0.005, 60.0)
+    assert (d.do_rdm, d.bit_length, d.dump_matrix_form_wf) == (1, 64, "w.txt")
+    assert (d.initial_adeterminant_bitstring, d.initial_bdeterminant_bitstring) == ("0011", "0101")
+    assert sbd_amd.generate_sbd_data(["prog"]) == sbd_amd.SBD()
+    with pytest.raises(ValueError):
+        sbd_amd.generate_sbd_data(["prog", "--block"])
+    with pytest.raises(sbd_amd.SbdError):
+        sbd_amd.generate_sbd_data(["prog", "--bdet_comm_size", "2"])
