@@ -325,7 +325,9 @@ def multiroot_section(args):
         got[label] = res
         out[label] = {"solves_per_s": round(len(res) / span, 3), "p50_s": round(pct(lat, 50), 4),
                       "p99_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3),
-                      "converged": sum(r["converged"] for r in res)}
+                      "converged": sum(r["converged"] for r in res),
+                      "steps_median": pct([r["steps"] for r in res], 50),
+                      "steps_fp32_median": pct([r["steps_fp32"] for r in res], 50)}
         log(f"multiroot {label}: p50 {pct(lat, 50):.2f}s p99 {pct(lat, 99):.2f}s "
             f"{len(res) / span:.2f} solves/s, converged {out[label]['converged']}/{len(res)}")
     err = np.array([[abs(a - b) for a, b in zip(x["energies"], y["energies"])]
@@ -484,6 +486,9 @@ def main():
     ap.add_argument("--mr-scale", type=float, default=0.1,
                     help="string-count scale of the multi-root requests (1.0 = 30000 x 25000)")
     ap.add_argument("--mr-max-it", type=int, default=60)
+    ap.add_argument("--sigma-kernel", default="auto",
+                    choices=["auto", "rows", "elements", "rows_inline", "split"],
+                    help="sigma kernel of the timed step (all give the same bits)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -519,6 +524,7 @@ def main():
     counts = m.table_counts()
     N = m.N
     log(f"tables {counts.tolist()} in {t_tables:.2f}s; N={N:.3e}")
+    m.set_sigma_kernel(args.sigma_kernel)
     t0 = time.time()
     hii = m.hdiag()
     torch.cuda.synchronize()
@@ -573,7 +579,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": B,
-                     "kernel": "k_sigma_rows<double, true, true, 1, true>"},
+                     "kernel": ("k_sigma_rows<double, true, true, 1, true>"
+                                if args.sigma_kernel == "auto" and args.workload == "sci36"
+                                else f"sigma kernel choice '{args.sigma_kernel}'")},
         "setup_s": {"tables": round(t_tables, 3), "hdiag": round(t_hdiag, 3)},
     }
 
