@@ -1130,7 +1130,7 @@ __global__ __launch_bounds__(kAbCols) void k_sigma_ab(SigmaArgs<VT> p, uint32_t 
   }
 }
 
-// Row-wise form of the dense pass (bases of up to kAbRowBlock * 8 beta
+// Row-wise form of the dense pass (bases of up to kAbRowBlock * 16 beta
 // strings): one 1024-thread workgroup per bra row (grid-stride over rows),
 // thread t owning columns t + 1024 j (j < C) with their sums in registers
 // across all of the row's alpha singles.  For each alpha single the ket row
@@ -1511,19 +1511,24 @@ static bool sigma_cached(const sbd_ctx* ctx) {
   return (ctx->ta.n_s == 0 || ctx->d_Ea) && (ctx->tb.n_s == 0 || ctx->d_Eb);
 }
 
+constexpr double kSplitDenseDoubles = 64.0;  // beta doubles per string
+
 // Whether a context outside the per-element rule takes the split form
 // (k_sigma_stream + k_sigma_bd) rather than the row kernel.  Both need the bra
 // rows in LDS.  By default the split form is taken when two or more rows fit
 // LDS together (measured faster in every such case: FP32 at 20,000 beta
-// strings, FCI norb 12, the reference's H2O); with one row per workgroup the
-// row kernel's two overlapped phases win.  Knob 4 forces it where it fits, 1
-// and 3 never take it.
+// strings, FCI norb 12, the reference's H2O) or the beta doubles are dense.
+// Knob 4 forces it where it fits, 1 and 3 never take it.
 template <typename VT>
 static bool sigma_use_split(const sbd_ctx* ctx) {
   if (sizeof(VT) * ctx->nb > (size_t)ctx->lds_max) return false;
   if (ctx->sigma_kernel == 4) return true;
   if (ctx->sigma_kernel != 0) return false;
-  return sigma_rows_for(ctx, sizeof(VT)) >= 2;
+  // one row per workgroup: the row kernel's overlapped phases win on sparse
+  // beta doubles (7 per string: 11.19 vs 11.78 ms), the split form on dense
+  // ones (906 per string, the reference's H2O 1e-6: 1622 vs 1678 ms)
+  return sigma_rows_for(ctx, sizeof(VT)) >= 2 ||
+         (double)ctx->tb.n_d / (double)ctx->nb >= kSplitDenseDoubles;
 }
 
 // The row kernel leaves the alpha-single x beta-single terms to k_sigma_ab
@@ -1564,8 +1569,8 @@ static int sigma_ab_rows_cfg(sbd_ctx* ctx, size_t elem, int* C, bool* dbuf, size
                              hipStream_t st) {
   *C = 0;
   const uint64_t per = (ctx->nb + kAbRowBlock - 1) / kAbRowBlock;
-  if (per > 8) return SBD_OK;  // wider rows take the column-block form
-  *C = per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : 8;
+  if (per > 16) return SBD_OK;  // wider rows take the column-block form
+  *C = per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : per <= 8 ? 8 : 16;
   const size_t row_bytes = (elem * ctx->nb + 15) & ~(size_t)15;
   const size_t buf = row_bytes + sizeof(double) * ctx->norb * ctx->norb;
   *dbuf = 2 * buf <= (size_t)ctx->lds_max;
@@ -1879,7 +1884,8 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
         case 1: SBD_AB_ROWS(1); break;
         case 2: SBD_AB_ROWS(2); break;
         case 4: SBD_AB_ROWS(4); break;
-        default: SBD_AB_ROWS(8); break;
+        case 8: SBD_AB_ROWS(8); break;
+        default: SBD_AB_ROWS(16); break;
       }
 #undef SBD_AB_ROWS
     } else {
@@ -1998,7 +2004,8 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
   (const void*)k_sigma_ab_rows<VT, 1, true>, (const void*)k_sigma_ab_rows<VT, 2, true>,           \
       (const void*)k_sigma_ab_rows<VT, 4, true>, (const void*)k_sigma_ab_rows<VT, 8, true>,       \
       (const void*)k_sigma_ab_rows<VT, 1, false>, (const void*)k_sigma_ab_rows<VT, 2, false>,     \
-      (const void*)k_sigma_ab_rows<VT, 4, false>, (const void*)k_sigma_ab_rows<VT, 8, false>
+      (const void*)k_sigma_ab_rows<VT, 4, false>, (const void*)k_sigma_ab_rows<VT, 8, false>, \
+      (const void*)k_sigma_ab_rows<VT, 16, true>, (const void*)k_sigma_ab_rows<VT, 16, false>
       SBD_AB_FNS(double), SBD_AB_FNS(float)
 #undef SBD_AB_FNS
   };
