@@ -379,7 +379,11 @@ int sbd_diag_footprint(const sbd_ctx* ctx, const sbd_options* opt, uint64_t* byt
  * does not take -- precision 1 (FP32 Davidson + FP64 sweep) or nroots > 1
  * (block Davidson, sbd_diag_roots) -- run whole on worker threads of their
  * own, each on its own stream, concurrently with the batches; their results
- * equal sbd_diag / sbd_diag_roots with the same options.  The reference has no
+ * equal sbd_diag / sbd_diag_roots with the same options.  sbd_server_create
+ * reserves one device arena of three quarters of mem_budget (less if the
+ * device has less free memory) from which the batched solves' Davidson vectors are
+ * handed out and to which they return, with no hipMalloc / hipFree per solve;
+ * tables and the whole-solve workers allocate per solve.  The reference has no
  * serving mode.  Server calls are thread-safe. */
 typedef struct sbd_server sbd_server;
 

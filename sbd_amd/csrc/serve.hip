@@ -23,11 +23,15 @@
 //     sbd_diag / sbd_diag_roots: concurrent with the batches and with each
 //     other, and equal to those calls.
 //
-// Finished solves publish their result at once and hand their buffers and
-// context to a reaper thread: hipFree waits for the device, and doing it on
-// the lanes would stall every lane behind it.  The memory returns to the
-// budget when it has been freed, and queued requests are admitted as memory
-// and slots allow, so their work joins the next round.
+// The batched solves' Davidson vectors are blocks of one device arena
+// reserved when the server is created (up to the memory budget), so admission
+// and retirement make no hipMalloc / hipFree for them.  Finished solves
+// publish their result at once and hand their block and context to a reaper
+// thread (the context's tables are freed with hipFree, which waits for the
+// device; doing it on the lanes would stall every lane behind it).  The memory
+// returns to the budget when it has been freed, and queued requests are
+// admitted as memory, arena space and slots allow, so their work joins the
+// next round.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +40,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <string>
@@ -54,6 +59,83 @@ constexpr int kSoloWorkers = 4;  // threads + streams of whole (FP32 / multi-roo
 
 double clock_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// The device memory of the batched solves' Davidson vectors: one allocation
+// made when the server is created, handed out first-fit in 2 MiB-aligned
+// blocks (one block per solve) and returned on completion, so admitting and
+// retiring a solve calls neither hipMalloc nor hipFree for its vectors.
+struct Arena {
+  static constexpr size_t kAlign = size_t(1) << 21;
+  char* base = nullptr;
+  size_t cap = 0;
+  std::map<size_t, size_t> free_;  // offset -> bytes, coalesced
+  std::mutex m;
+
+  static size_t round_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+  // reserve up to want bytes (less when the device has less free memory)
+  void init(size_t want) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) want = std::min(want, fr / 10 * 9);
+    for (size_t b = round_up(want); b >= (size_t(1) << 30); b = round_up(b / 2)) {
+      if (hipMalloc(&base, b) == hipSuccess) {
+        cap = b;
+        free_[0] = b;
+        return;
+      }
+      (void)hipGetLastError();
+      base = nullptr;
+    }
+  }
+  size_t largest() {
+    std::lock_guard<std::mutex> g(m);
+    size_t mx = 0;
+    for (auto& f : free_) mx = std::max(mx, f.second);
+    return mx;
+  }
+  void* alloc(size_t bytes) {
+    bytes = round_up(bytes);
+    std::lock_guard<std::mutex> g(m);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + bytes] = rest;
+      return base + off;
+    }
+    return nullptr;
+  }
+  void release(void* p, size_t bytes) {
+    bytes = round_up(bytes);
+    std::lock_guard<std::mutex> g(m);
+    size_t off = (size_t)((char*)p - base);
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + bytes == next->first) {
+      bytes += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        off = prev->first;
+        bytes += prev->second;
+        free_.erase(prev);
+      }
+    }
+    free_[off] = bytes;
+  }
+  void destroy() {
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    cap = 0;
+    free_.clear();
+  }
+};
+
+// bytes of one batched solve's vectors (C, HC, spare, hii) in the arena
+size_t solve_block_bytes(uint64_t N, int nb) {
+  const size_t v = (size_t)(8 * N + 255) / 256 * 256;
+  return v * (size_t)(2 * nb + 2);
 }
 
 struct Request {
@@ -76,6 +158,9 @@ struct Solve {
   std::vector<double*> C, HC;
   double* spare = nullptr;
   double* hii = nullptr;
+  Arena* arena = nullptr;  // set when the vectors are a block of the arena
+  void* block = nullptr;
+  size_t block_bytes = 0;
   DavidsonJob<double> job;
   double t_admit = 0.0;
   double cost = 0.0;  // estimated work of one step (terms), for lane balancing
@@ -83,12 +168,17 @@ struct Solve {
   int status = SBD_OK;
 
   void release() {
-    for (double* p : C)
-      if (p) (void)hipFree(p);
-    for (double* p : HC)
-      if (p) (void)hipFree(p);
-    if (spare) (void)hipFree(spare);
-    if (hii) (void)hipFree(hii);
+    if (block) {
+      arena->release(block, block_bytes);
+      block = nullptr;
+    } else {
+      for (double* p : C)
+        if (p) (void)hipFree(p);
+      for (double* p : HC)
+        if (p) (void)hipFree(p);
+      if (spare) (void)hipFree(spare);
+      if (hii) (void)hipFree(hii);
+    }
     C.clear();
     HC.clear();
     spare = hii = nullptr;
@@ -192,6 +282,7 @@ struct sbd_server {
   uint64_t next_id = 1;
   std::thread admitter, reaper;
   hipStream_t st_admit = nullptr;
+  Arena arena;
   double t0 = 0.0;
   uint64_t rounds = 0, peak_active = 0;
   std::string last_error;
@@ -273,12 +364,28 @@ struct sbd_server {
     const int nb = q.opt.max_nb;
     s->C.assign(nb, nullptr);
     s->HC.assign(nb, nullptr);
-    for (int k = 0; k < nb; k++) {
-      if (hipMalloc(&s->C[k], 8 * N) != hipSuccess || hipMalloc(&s->HC[k], 8 * N) != hipSuccess)
+    const size_t bytes = solve_block_bytes(N, nb);
+    if (arena.base) {
+      s->block = arena.alloc(bytes);
+      if (!s->block) return SBD_ERR_MEMORY;
+      s->arena = &arena;
+      s->block_bytes = bytes;
+      const size_t v = bytes / (size_t)(2 * nb + 2);
+      char* b = (char*)s->block;
+      for (int k = 0; k < nb; k++) {
+        s->C[k] = (double*)(b + (size_t)(2 * k) * v);
+        s->HC[k] = (double*)(b + (size_t)(2 * k + 1) * v);
+      }
+      s->spare = (double*)(b + (size_t)(2 * nb) * v);
+      s->hii = (double*)(b + (size_t)(2 * nb + 1) * v);
+    } else {
+      for (int k = 0; k < nb; k++) {
+        if (hipMalloc(&s->C[k], 8 * N) != hipSuccess || hipMalloc(&s->HC[k], 8 * N) != hipSuccess)
+          return SBD_ERR_MEMORY;
+      }
+      if (hipMalloc(&s->spare, 8 * N) != hipSuccess || hipMalloc(&s->hii, 8 * N) != hipSuccess)
         return SBD_ERR_MEMORY;
     }
-    if (hipMalloc(&s->spare, 8 * N) != hipSuccess || hipMalloc(&s->hii, 8 * N) != hipSuccess)
-      return SBD_ERR_MEMORY;
     // the diagonal and the start vector as sbd_diag makes them
     // (BasisInitVector, davidson.h:144-184), on the admission stream
     s->ctx->stream = st_admit;
@@ -307,7 +414,12 @@ struct sbd_server {
           const uint64_t need = pending[k].bytes;
           const int active = running + preparing + queued;
           const bool slot = active < max_active;
-          const bool mem = used + need <= budget || (active == 0 && reaping == 0);
+          const bool idle = active == 0 && reaping == 0;
+          bool mem = used + need <= budget || idle;
+          // a batched solve's vectors must also find a block of the arena
+          if (mem && !idle && arena.base && pending[k].batched())
+            mem = arena.largest() >=
+                  solve_block_bytes(pending[k].a.size() * pending[k].b.size(), pending[k].opt.max_nb);
           return slot && mem;
         });
         if (pending.empty()) return;  // stopping
@@ -599,6 +711,11 @@ sbd_server* sbd_server_create(uint64_t mem_budget, int max_active, int* status) 
     ok = ok && hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) == hipSuccess;
   for (Solo& w : sv->solos)
     ok = ok && hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking) == hipSuccess;
+  // the batched solves' vectors come from one arena of three quarters of the
+  // budget (the rest, counted in the same budget, is the contexts' tables;
+  // less when the device has less free memory; without an arena the vectors
+  // are allocated per solve)
+  if (ok) sv->arena.init(mem_budget / 4 * 3);
   if (!ok) {
     for (Lane& l : sv->lanes)
       if (l.st) (void)hipStreamDestroy(l.st);
@@ -719,6 +836,7 @@ void sbd_server_destroy(sbd_server* sv) {
   for (Solo& w : sv->solos)
     if (w.st) (void)hipStreamDestroy(w.st);
   if (sv->st_admit) (void)hipStreamDestroy(sv->st_admit);
+  sv->arena.destroy();
   delete sv;
 }
 
