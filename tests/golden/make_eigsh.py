@@ -15,7 +15,9 @@ from scipy.sparse.linalg import LinearOperator, eigsh
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 from sbd_amd import synth  # noqa: E402
-from oracle import oracle as O  # noqa: E402
+from oracle import oracle as O  # noqa: E402,F401
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from make_energies import system_of  # noqa: E402
 
 path = os.path.join(ROOT, "tests", "golden", "energies.json")
 data = json.load(open(path))
@@ -23,8 +25,7 @@ want = sys.argv[1:] or [k for k, d in data.items() if "energy_eigsh" not in d]
 for key, d in data.items():
     if key not in want:
         continue
-    w = synth.make_workload(d["workload"], d["scale"])
-    S = O.System.from_workload(w)
+    S = system_of(d["workload"], d["scale"])
     hii = S.hdiag()
     n_mv = [0]
 

@@ -96,6 +96,8 @@ CASES = [
     ("sci58_small", lambda: make_pair(synth_case("sci58", 0.01))),
     ("h2o_1em3", lambda: make_pair_fcidump("h2o", "h2o-1em3-alpha.txt.gz")),
     ("n2_1em3", lambda: make_pair_fcidump("n2", "1em3-alpha.txt.gz")),
+    # the reference's Fe4S4 cluster: norb 36, two-word determinants
+    ("fe4s4", lambda: make_pair_fcidump("fe4s4", "alpha.txt.gz")),
 ]
 
 
@@ -209,6 +211,30 @@ def test_reference_energies(torch, mol, alpha, E_known):
     assert rep["converged"] == 1
     assert abs(E - E_known) < 6e-9, (E, E_known)
     m.close()
+
+
+def test_fe4s4_energy(torch):
+    """The reference's Fe4S4 cluster (fcidump_Fe4S4.txt with AlphaDets.txt for
+    both spins), quoted as -326.70 Ha (tests/functionality/
+    test_gpu_correctness.cc:221): the FP64 and the mixed-precision solves land
+    within 1e-9 Ha of the oracle's converged energy (tests/golden/energies.json
+    "fe4s4", within 2e-13 of ARPACK) and round to the quoted value."""
+    import json
+    import sbd_amd
+    say("reference energy fe4s4")
+    gold = json.load(open(os.path.join(GOLDEN, "energies.json")))["fe4s4"]
+    ref, m = make_pair_fcidump("fe4s4", "alpha.txt.gz")
+    assert m.N == gold["N"]
+    E64, _, r64 = m.diag(sbd_amd.SBD(max_it=20, max_nb=10, eps=1e-8))
+    E32, _, r32 = m.diag(sbd_amd.SBD(max_it=20, max_nb=10, eps=1e-8, precision=1,
+                                     fp32_eps=1e-4))
+    m.close()
+    say(f"fe4s4: E64 {E64:.12f} ({r64['steps']} steps), mixed {E32:.12f}, "
+        f"oracle {gold['energy']:.12f}")
+    assert r64["converged"] == 1 and r32["converged"] == 1
+    assert abs(E64 - gold["energy"]) < 1e-9, (E64, gold["energy"])
+    assert abs(E32 - gold["energy"]) < 1e-9, (E32, gold["energy"])
+    assert round(E64, 2) == -326.70
 
 
 def test_mixed_precision_energy(torch):

@@ -282,6 +282,31 @@ def test_oracle_reference_energies(O, mol, alpha, E_known):
     assert abs(E - E_known) < 6e-9, (E, E_known)
 
 
+def test_oracle_fe4s4_energy(O):
+    """The Fe4S4 cluster the reference ships (fcidump_Fe4S4.txt, AlphaDets.txt:
+    norb 36, 27 + 27 electrons, 244 strings, 59,536 determinants), whose ground
+    energy its tests quote to two decimals as -326.70 Ha
+    (tests/functionality/test_gpu_correctness.cc:221).  Twenty Davidson steps
+    from the Hartree-Fock determinant are within 1e-7 Ha of the committed
+    converged energy (tests/golden/energies.json "fe4s4", itself within 2e-13
+    of ARPACK on the oracle's sigma) and round to the quoted value."""
+    hdr, vals, idx = O.load_fcidump(os.path.join(GOLDEN, "fe4s4", "fcidump.txt.gz"))
+    L, _, I0, I1, I2, Dm, Xm = O.setup_integrals(hdr, vals, idx)
+    assert L == 36 and int(hdr["NELEC"]) == 54
+    a = O.load_alpha_dets(os.path.join(GOLDEN, "fe4s4", "alpha.txt.gz"), L)
+    assert len(a) == 244
+    S = O.System(L, a, a, I0, I1, I2, Dm, Xm)
+    hii = S.hdiag()
+    W0 = np.zeros(S.N)
+    W0[0] = 1.0
+    W, hist = S.davidson(hii, W0, 2, 10, 1e-8)
+    E = S.energy(hii, W)
+    gold = json.load(open(os.path.join(GOLDEN, "energies.json")))["fe4s4"]
+    assert gold["N"] == S.N == 59536
+    assert gold["energy"] - 1e-9 <= E < gold["energy"] + 1e-7, (E, gold["energy"])
+    assert round(E, 2) == -326.70
+
+
 def test_synthetic_generator_is_pinned():
     """The seeded generator is part of the benchmark definition: its outputs are
     pinned by digests committed in tests/golden/synth_digests.json."""
