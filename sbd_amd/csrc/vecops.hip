@@ -161,10 +161,14 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
   for (int j = 0; j < MAXK; j++) ot[j] = oh[j] = 0.0;
   // FP64: all 2k chunks are held until t is known, so each is read once.
   // FP32: w and r are accumulated as the chunks arrive and, once t is known,
-  // the 2k chunks are read again (from L2: this thread has just read them,
-  // and FP32 chunks are small enough to stay there) for the overlaps, so no
-  // chunk is held across the division and the registers leave room for
-  // several waves per SIMD.  Both give the same sums in the same order.
+  // the chunks are taken again for the overlaps: the first kStage pairs
+  // (C_j, HC_j) from LDS, where each thread parked them, the others read
+  // again newest first (the lines most likely still in L2).  No chunk is held
+  // in registers across the division, so four waves fit per SIMD (the 32 KiB
+  // of LDS allows four workgroups per CU as well).  Every accumulator takes
+  // its terms in the same order in both forms, so the sums are the same.
+  constexpr int kStage = 4;
+  __shared__ Chunk<VT, chunk_of<VT>()> stage[sizeof(VT) == 4 ? 2 * kStage : 1][kVecBlock];
   auto hold = [&]<int W>(uint64_t i) {
     double w[W], r[W];
     Chunk<VT, W> cv[MAXK], hv[MAXK];
@@ -224,6 +228,12 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
       if (j < k) {
         const Chunk<VT, W> cv = ldc<W>(C.p[j], i);
         const Chunk<VT, W> hv = ldc<W>(HC.p[j], i);
+        if constexpr (W == chunk_of<VT>() && sizeof(VT) == 4) {
+          if (j < kStage) {
+            stage[2 * j][threadIdx.x] = cv;
+            stage[2 * j + 1][threadIdx.x] = hv;
+          }
+        }
 #pragma unroll
         for (int e = 0; e < W; e++) {
           if (j == 0) {
@@ -258,10 +268,21 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
     uint64_t i2 = i;
     asm volatile("" : "+v"(i2));
 #pragma unroll
-    for (int j = 0; j < MAXK; j++) {
+    for (int j = MAXK - 1; j >= 0; j--) {
       if (j < k) {
-        const Chunk<VT, W> cv = ldc<W>(C.p[j], i2);
-        const Chunk<VT, W> hv = ldc<W>(HC.p[j], i2);
+        Chunk<VT, W> cv, hv;
+        bool staged = false;
+        if constexpr (W == chunk_of<VT>() && sizeof(VT) == 4) {
+          if (j < kStage) {
+            cv = stage[2 * j][threadIdx.x];
+            hv = stage[2 * j + 1][threadIdx.x];
+            staged = true;
+          }
+        }
+        if (!staged) {
+          cv = ldc<W>(C.p[j], i2);
+          hv = ldc<W>(HC.p[j], i2);
+        }
 #pragma unroll
         for (int e = 0; e < W; e++) {
           ot[j] += (double)cv.v[e] * tv[e];
