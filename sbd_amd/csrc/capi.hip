@@ -873,9 +873,10 @@ static int diag_impl(sbd_ctx* ctx, const sbd_options* opt, const double* W_init,
   if (rc) return rc;
   R->t_tables = now_s() - t0;
   // Device memory is held to what each phase needs: the FP32 phase holds its
-  // vectors and FP32 diagonal only, and frees them (but W) before the FP64
-  // diagonal is made; the final <W, H W> takes its H W buffer once the
-  // subspace vectors are gone.
+  // vectors and FP32 diagonal (plus the FP64 diagonal when the full FP32
+  // subspace still fits beside it), and frees them (but W) before the FP64
+  // phase; the final <W, H W> takes its H W buffer once the subspace vectors
+  // are gone.
   DevBuf<double> W, hii;
   double E = 0.0;
   t0 = now_s();
@@ -891,7 +892,24 @@ static int diag_impl(sbd_ctx* ctx, const sbd_options* opt, const double* W_init,
       rc = start_vector<float>(ctx, opt, W_init, W32.p, unit);
       if (rc) return rc;
       const double th = now_s();
-      rc = launch_hdiag_f32(ctx, hii32.p, st);
+      // The FP32 diagonal is the FP64 one rounded once (k_hdiag forms it in
+      // FP64 either way), so when HBM holds the FP64 diagonal beside the full
+      // FP32 subspace it is made once here, rounded, and kept for the FP64
+      // phase; otherwise each phase makes its own.
+      size_t fr = 0, tot = 0;
+      bool keep64 = hipMemGetInfo(&fr, &tot) == hipSuccess &&
+                    (double)fr >= 8.0 * N + 2.0 * opt->max_nb * 4.0 * N + 0.02 * (double)tot;
+      if (keep64 && hipMalloc(&hii.p, 8 * N) != hipSuccess) {
+        (void)hipGetLastError();  // not an error: the FP32 diagonal is made on its own
+        hii.p = nullptr;
+        keep64 = false;
+      }
+      if (keep64) {
+        rc = launch_hdiag(ctx, hii.p, st);
+        if (!rc) rc = vec_convert<double, float>(ctx, hii.p, hii32.p, N, st);
+      } else {
+        rc = launch_hdiag_f32(ctx, hii32.p, st);
+      }
       if (rc) return rc;
       SBD_HIP_TRY(ctx, hipStreamSynchronize(st));
       R->t_hdiag += now_s() - th;
@@ -921,9 +939,9 @@ static int diag_impl(sbd_ctx* ctx, const sbd_options* opt, const double* W_init,
     rc = start_vector<double>(ctx, opt, W_init, W.p, unit);
     if (rc) return rc;
   }
-  if (hipMalloc(&hii.p, 8 * N) != hipSuccess)
-    return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
-  {
+  if (!hii.p) {
+    if (hipMalloc(&hii.p, 8 * N) != hipSuccess)
+      return set_err(ctx, SBD_ERR_MEMORY, "device allocation failed");
     const double th = now_s();
     rc = launch_hdiag(ctx, hii.p, st);
     if (rc) return rc;

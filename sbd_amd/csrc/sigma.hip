@@ -1394,17 +1394,31 @@ __global__ __launch_bounds__(kElemBlock) void k_sigma_elems(const SigmaArgs<VT> 
 }
 
 // Sum of n partials in a fixed order: thread t adds partials t, t + 1024, ...
-// (compensated), then the 1024 thread sums are combined in a fixed tree.
+// (compensated), then the 1024 thread sums are combined in a fixed tree.  The
+// loads are issued eight at a time ahead of the additions, which keep their
+// order.
 constexpr int kSumBlock = 1024;
-__global__ __launch_bounds__(kSumBlock) void k_sum_fixed(const double* __restrict__ part,
-                                                         uint64_t n, double* __restrict__ out) {
+__device__ inline void sum_fixed_block(const double* __restrict__ part, uint64_t n,
+                                       double* __restrict__ out) {
   __shared__ double red[kSumBlock];
+  constexpr int U = 8;
   double s = 0.0, c = 0.0;
-  for (uint64_t i = threadIdx.x; i < n; i += kSumBlock) {
-    const double y = part[i] - c;
-    const double t = s + y;
-    c = (t - s) - y;
-    s = t;
+  for (uint64_t i0 = threadIdx.x; i0 < n; i0 += (uint64_t)U * kSumBlock) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t i = i0 + (uint64_t)u * kSumBlock;
+      v[u] = i < n ? part[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (i0 + (uint64_t)u * kSumBlock < n) {
+        const double y = v[u] - c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+      }
+    }
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -1413,6 +1427,10 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_fixed(const double* __restric
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0];
+}
+__global__ __launch_bounds__(kSumBlock) void k_sum_fixed(const double* __restrict__ part,
+                                                         uint64_t n, double* __restrict__ out) {
+  sum_fixed_block(part, n, out);
 }
 
 int build_single_cache(sbd_ctx* ctx, hipStream_t st) {
@@ -1446,22 +1464,8 @@ struct SumJob {
 };
 // k_sum_fixed for several jobs, one block each (same arithmetic)
 __global__ __launch_bounds__(kSumBlock) void k_sum_fixed_batch(const SumJob* __restrict__ jobs) {
-  __shared__ double red[kSumBlock];
   const SumJob& J = jobs[blockIdx.x];
-  double s = 0.0, c = 0.0;
-  for (uint64_t i = threadIdx.x; i < J.n; i += kSumBlock) {
-    const double y = J.part[i] - c;
-    const double t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = kSumBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) J.out[0] = red[0];
+  sum_fixed_block(J.part, J.n, J.out);
 }
 
 template <typename VT>
