@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kVecBlock) void k_multidot(VecPtrs<VT> X, int m,
 // sums: [0] |r|^2, [1] |w|^2, [2] |t|^2, [3, 3+k) <C_j, t>, [3+MAXK, 3+MAXK+k)
 // <HC_j, t> (t as stored); the last group gives the next projected-matrix
 // column by symmetry of H, so no separate overlap pass follows the sigma.
-template <typename VT, int MAXK>
+template <typename VT, int MAXK, bool kHold>
 __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>& HC, int k,
                                  const Coefs& u, double E0, const VT* __restrict__ dii,
                                  VT* __restrict__ t, uint64_t n, bool packed,
@@ -159,16 +159,19 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
   double ot[MAXK], oh[MAXK];
 #pragma unroll
   for (int j = 0; j < MAXK; j++) ot[j] = oh[j] = 0.0;
-  // FP64: all 2k chunks are held until t is known, so each is read once.
-  // FP32: w and r are accumulated as the chunks arrive and, once t is known,
-  // the chunks are taken again for the overlaps: the first kStage pairs
-  // (C_j, HC_j) from LDS, where each thread parked them, the others read
-  // again newest first (the lines most likely still in L2).  No chunk is held
-  // in registers across the division, so four waves fit per SIMD (the 32 KiB
-  // of LDS allows four workgroups per CU as well).  Every accumulator takes
-  // its terms in the same order in both forms, so the sums are the same.
+  // kHold (FP64 single launches up to 12 vectors per side): all 2k chunks
+  // are held in registers until t is known, so each is read once.
+  // Otherwise (FP32, wider FP64 subspaces and the batched launches, where
+  // holding them would spill): w and r are accumulated as the chunks arrive
+  // and, once t is known, the chunks are taken again for the overlaps: the
+  // first kStage pairs (C_j, HC_j) from LDS, where each thread parked them,
+  // the others read again newest first (the lines most likely still in L2).
+  // No chunk is held in registers across the division, so four waves fit per
+  // SIMD (the 32 KiB of LDS allows four workgroups per CU as well).  Every
+  // accumulator takes its terms in the same order in both forms, so the sums
+  // are the same.
   constexpr int kStage = 4;
-  __shared__ Chunk<VT, chunk_of<VT>()> stage[sizeof(VT) == 4 ? 2 * kStage : 1][kVecBlock];
+  __shared__ Chunk<VT, chunk_of<VT>()> stage[kHold ? 1 : 2 * kStage][kVecBlock];
   auto hold = [&]<int W>(uint64_t i) {
     double w[W], r[W];
     Chunk<VT, W> cv[MAXK], hv[MAXK];
@@ -228,7 +231,7 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
       if (j < k) {
         const Chunk<VT, W> cv = ldc<W>(C.p[j], i);
         const Chunk<VT, W> hv = ldc<W>(HC.p[j], i);
-        if constexpr (W == chunk_of<VT>() && sizeof(VT) == 4) {
+        if constexpr (W == chunk_of<VT>() && !kHold) {
           if (j < kStage) {
             stage[2 * j][threadIdx.x] = cv;
             stage[2 * j + 1][threadIdx.x] = hv;
@@ -272,7 +275,7 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
       if (j < k) {
         Chunk<VT, W> cv, hv;
         bool staged = false;
-        if constexpr (W == chunk_of<VT>() && sizeof(VT) == 4) {
+        if constexpr (W == chunk_of<VT>() && !kHold) {
           if (j < kStage) {
             cv = stage[2 * j][threadIdx.x];
             hv = stage[2 * j + 1][threadIdx.x];
@@ -291,7 +294,7 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
       }
     }
   };
-  if constexpr (sizeof(VT) == 8) for_chunks<VT>(g, n, packed, hold);
+  if constexpr (kHold) for_chunks<VT>(g, n, packed, hold);
   else for_chunks<VT>(g, n, packed, reread);
   double v[2 * MAXK + 3];
   v[0] = rr.s;
@@ -312,7 +315,8 @@ __global__ __launch_bounds__(kVecBlock) void k_ritz_precond(VecPtrs<VT> C, VecPt
                                                             VT* __restrict__ t, uint64_t n,
                                                             bool packed,
                                                             double* __restrict__ part) {
-  ritz_body<VT, MAXK>(launch_grid(), C, HC, k, u, E0, dii, t, n, packed, part);
+  ritz_body<VT, MAXK, sizeof(VT) == 8 && MAXK <= 12>(launch_grid(), C, HC, k, u, E0, dii, t, n,
+                                                      packed, part);
 }
 
 // Classical Gram-Schmidt update t = scale * (t - sum o_j C_j); sums |t|^2.
@@ -402,7 +406,7 @@ template <typename VT, int MAXK>
 __global__ __launch_bounds__(kVecBlock) void k_ritz_batch(const RitzJob<VT>* __restrict__ jobs,
                                                           int njobs) {
   const RitzJob<VT>& J = find_job(jobs, njobs);
-  ritz_body<VT, MAXK>(Grid{blockIdx.x - J.blk0, J.nblk}, J.C, J.HC, J.k, J.u, J.E0, J.dii, J.t,
+  ritz_body<VT, MAXK, false>(Grid{blockIdx.x - J.blk0, J.nblk}, J.C, J.HC, J.k, J.u, J.E0, J.dii, J.t,
                       J.n, J.packed != 0, J.part);
 }
 
