@@ -681,8 +681,14 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_batch(const SigmaJob<VT>* _
     if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
     else hi = mid - 1;
   }
-  const SigmaJob<VT>& J = jobs[lo];
-  sigma_rows_body<VT, true, true, kRows, true>(J.p, blockIdx.x - J.blk0, J.ngrp, J.dot_part);
+  // the job's arguments as a uniform local copy (kept in scalar registers like
+  // a kernel's own arguments; read through a reference into the job table they
+  // were held in vector registers and spilled)
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const SigmaArgs<VT> p = jobs[lo].p;
+  const uint32_t blk0 = jobs[lo].blk0, ngrp = jobs[lo].ngrp;
+  double* const dot_part = jobs[lo].dot_part;
+  sigma_rows_body<VT, true, true, kRows, true>(p, blockIdx.x - blk0, ngrp, dot_part);
 }
 
 // ---------------------------------------------------------------- split sigma
@@ -1028,8 +1034,12 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_bd_batch(const SigmaJob<VT>
     if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
     else hi = mid - 1;
   }
-  const SigmaJob<VT>& Jb = jobs[lo];
-  sigma_bd_body<VT, kRows, true, true>(Jb.p, blockIdx.x - Jb.blk0, Jb.ngrp, Jb.dot_part);
+  // a uniform local copy of the job's arguments, as in k_sigma_batch
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const SigmaArgs<VT> p = jobs[lo].p;
+  const uint32_t blk0 = jobs[lo].blk0, ngrp = jobs[lo].ngrp;
+  double* const dot_part = jobs[lo].dot_part;
+  sigma_bd_body<VT, kRows, true, true>(p, blockIdx.x - blk0, ngrp, dot_part);
 }
 
 // ---------------------------------------------------------------- dense alpha x beta
