@@ -327,7 +327,7 @@ int davidson_core(sbd_ctx* ctx, const VT* dii, VT* W, int max_it, int nb, double
         if (rc) break;
         const auto Cc = ptrs(job.C, k), HCc = ptrs(job.HC, k);
         rc = vec_ritz_precond<VT>(ctx, Cc.data(), HCc.data(), k, job.ritz_u(), job.E0, dii, job.t(),
-                                  N, job.ritz_sums(), st);
+                                  N, job.ritz_sums(), st, job.ritz_restart());
         if (rc) break;
         const int ib = job.ib, it = job.it;
         job.ritz_done();

@@ -15,6 +15,9 @@
 //   kGs     : classical Gram-Schmidt of t against C[0..ib] -> gs_done
 //   kDots   : (only when 99.9% of |t| cancels) <C_j, t>    -> dots_done
 //   kLincomb: restart vector sum u C into the spare buffer  -> lincomb_done
+//             (not reached from a Ritz pass run with ritz_restart(): at the
+//             last step of a cycle that pass itself stores sum u C in the
+//             spare buffer, t's place, with its |.|^2 in ritz sum 2)
 //   kScale  : renormalise the restart vector                 -> scale_done
 //   kDone
 #pragma once
@@ -75,6 +78,8 @@ class DavidsonJob {
   VT* sigma_out() const { return HC[ib]; }
   const double* ritz_u() const { return u.data(); }
   VT* t() const { return ib < nb - 1 ? C[ib + 1] : spare; }
+  // the Ritz pass of this step writes the restart vector into t() (the spare)
+  bool ritz_restart() const { return ib == nb - 1; }
   double* ritz_sums() { return sums.data(); }  // 3 + 2k values
   const double* gs_coef() const { return gs_second ? o2.data() : sums.data() + 3; }
   double gs_scale() const { return gs_sc; }
@@ -126,6 +131,8 @@ class DavidsonJob {
       op = kGs;
     } else {
       end_of_block();
+      // the pass stored the restart vector sum u C in the spare buffer
+      if (op == kLincomb) lincomb_done(sums[2]);
     }
     return false;
   }

@@ -276,8 +276,11 @@ int vec_multidot(sbd_ctx* ctx, const VT* const* X, int m, const VT* y, uint64_t 
                  hipStream_t st);
 // out: [0] |r|^2, [1] |w|^2, [2] |t|^2, [3, 3+k) <C_j, t>, [3+k, 3+2k) <HC_j, t>
 template <typename VT>
+// wout: the last step of a restart cycle; t receives the restart vector
+// sum u_j C_j and out[2] its |.|^2 (as vec_lincomb forms them), no overlaps
 int vec_ritz_precond(sbd_ctx* ctx, const VT* const* C, const VT* const* HC, int k, const double* u,
-                     double E0, const VT* dii, VT* t, uint64_t n, double* out, hipStream_t st);
+                     double E0, const VT* dii, VT* t, uint64_t n, double* out, hipStream_t st,
+                     bool wout = false);
 template <typename VT>
 int vec_gs_update(sbd_ctx* ctx, const VT* const* C, int k, const double* coef, VT* t, double scale,
                   uint64_t n, double* out_norm2, hipStream_t st);
@@ -357,6 +360,7 @@ struct RitzArgs {
   const VT* dii;
   VT* t;
   uint64_t n;
+  bool wout = false;  // restart-vector form (see vec_ritz_precond)
 };
 // out[j]: 3 + 2 k_j sums in the layout of vec_ritz_precond
 template <typename VT>

@@ -597,7 +597,7 @@ struct sbd_server {
         cp[q].assign(s->job.C.begin(), s->job.C.begin() + k);
         hp[q].assign(s->job.HC.begin(), s->job.HC.begin() + k);
         args.push_back(RitzArgs<double>{cp[q].data(), hp[q].data(), k, s->job.ritz_u(), s->job.E0,
-                                        s->hii, s->job.t(), s->ctx->N});
+                                        s->hii, s->job.t(), s->ctx->N, s->job.ritz_restart()});
         outs.push_back(s->job.ritz_sums());
       }
       if (!who.empty()) {

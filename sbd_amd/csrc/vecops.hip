@@ -151,7 +151,7 @@ template <typename VT, int MAXK, bool kHold>
 __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>& HC, int k,
                                  const Coefs& u, double E0, const VT* __restrict__ dii,
                                  VT* __restrict__ t, uint64_t n, bool packed,
-                                 double* __restrict__ part) {
+                                 double* __restrict__ part, bool wout) {
   constexpr double eps_reg = 1.0e-12;
   // norms compensated; the 2k overlaps as plain FP64 per-thread sums (the
   // register budget of 2k compensated sums would cut the waves in flight)
@@ -294,7 +294,46 @@ __device__ inline void ritz_body(Grid g, const VecPtrs<VT>& C, const VecPtrs<VT>
       }
     }
   };
-  if constexpr (kHold) for_chunks<VT>(g, n, packed, hold);
+  // wout (the last step of a restart cycle, where t has no slot in the
+  // subspace): the pass stores the restart vector w = sum u_j C_j in t's place
+  // and sums |w|^2 of the stored values in the place of |t|^2, exactly as
+  // k_lincomb forms and sums it (same chunks, same order), with |r|^2 and
+  // |w|^2 as in the full pass; the overlaps are not formed
+  auto restart = [&]<int W>(uint64_t i) {
+    double w[W], r[W];
+#pragma unroll
+    for (int j = 0; j < MAXK; j++) {
+      if (j < k) {
+        const Chunk<VT, W> cv = ldc<W>(C.p[j], i);
+        const Chunk<VT, W> hv = ldc<W>(HC.p[j], i);
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+          if (j == 0) {
+            w[e] = (double)cv.v[e] * u.c[0];
+            r[e] = (double)hv.v[e] * (-1.0 * u.c[0]);
+          } else {
+            w[e] += (double)cv.v[e] * u.c[j];
+            r[e] += (double)hv.v[e] * (-1.0 * u.c[j]);
+          }
+        }
+      }
+    }
+    Chunk<VT, W> ws;
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      r[e] += E0 * w[e];
+      ws.v[e] = (VT)w[e];
+    }
+    stc<W>(t, i, ws);
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      rr.add(r[e] * r[e]);
+      ww.add(w[e] * w[e]);
+      tt.add((double)ws.v[e] * (double)ws.v[e]);
+    }
+  };
+  if (wout) for_chunks<VT>(g, n, packed, restart);
+  else if constexpr (kHold) for_chunks<VT>(g, n, packed, hold);
   else for_chunks<VT>(g, n, packed, reread);
   double v[2 * MAXK + 3];
   v[0] = rr.s;
@@ -314,9 +353,10 @@ __global__ __launch_bounds__(kVecBlock) void k_ritz_precond(VecPtrs<VT> C, VecPt
                                                             const VT* __restrict__ dii,
                                                             VT* __restrict__ t, uint64_t n,
                                                             bool packed,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part,
+                                                            bool wout) {
   ritz_body<VT, MAXK, sizeof(VT) == 8 && MAXK <= 12>(launch_grid(), C, HC, k, u, E0, dii, t, n,
-                                                      packed, part);
+                                                      packed, part, wout);
 }
 
 // Classical Gram-Schmidt update t = scale * (t - sum o_j C_j); sums |t|^2.
@@ -370,7 +410,7 @@ struct RitzJob {
   VT* t;
   uint64_t n;
   double E0;
-  int k, packed;
+  int k, packed, wout;
   uint32_t blk0, nblk;
   double* part;
 };
@@ -407,7 +447,7 @@ __global__ __launch_bounds__(kVecBlock) void k_ritz_batch(const RitzJob<VT>* __r
                                                           int njobs) {
   const RitzJob<VT>& J = find_job(jobs, njobs);
   ritz_body<VT, MAXK, false>(Grid{blockIdx.x - J.blk0, J.nblk}, J.C, J.HC, J.k, J.u, J.E0, J.dii, J.t,
-                      J.n, J.packed != 0, J.part);
+                             J.n, J.packed != 0, J.part, J.wout != 0);
 }
 
 template <typename VT, int MAXK>
@@ -605,14 +645,15 @@ int vec_multidot(sbd_ctx* ctx, const VT* const* X, int m, const VT* y, uint64_t 
 
 template <typename VT>
 int vec_ritz_precond(sbd_ctx* ctx, const VT* const* C, const VT* const* HC, int k, const double* u,
-                     double E0, const VT* dii, VT* t, uint64_t n, double* out, hipStream_t st) {
+                     double E0, const VT* dii, VT* t, uint64_t n, double* out, hipStream_t st,
+                     bool wout) {
   if (k < 1 || k > kMaxVecs) return SBD_ERR_ARG;
   int rc = ensure_scratch(ctx);
   if (rc) return rc;
   const int g = grid_for(n);
   const bool packed = aligned16(C, k) && aligned16(HC, k) && aligned16<VT>({dii, t});
   SBD_DISPATCH_K(k, k_ritz_precond, dim3(g), dim3(kVecBlock), 0, st, pack(C, k), pack(HC, k), k,
-                 packc(u, k), E0, dii, t, n, packed, ctx->d_partials);
+                 packc(u, k), E0, dii, t, n, packed, ctx->d_partials, wout);
   SBD_HIP_TRY(ctx, hipGetLastError());
   const int K = size_class(k);
   std::vector<double> res(2 * K + 3);
@@ -764,6 +805,7 @@ int vec_ritz_batch(BatchScratch& sc, const RitzArgs<VT>* a, int njobs, double* c
     J.E0 = x.E0;
     J.k = x.k;
     J.packed = aligned16(x.C, x.k) && aligned16(x.HC, x.k) && aligned16<VT>({x.dii, x.t});
+    J.wout = x.wout ? 1 : 0;
     J.nblk = (uint32_t)grid_for(x.n);
     J.blk0 = blk;
     blk += J.nblk;
@@ -866,7 +908,7 @@ template int vec_gs_batch<double>(BatchScratch&, const GsArgs<double>*, int, dou
                                 hipStream_t);                                                 \
   template int vec_ritz_precond<VT>(sbd_ctx*, const VT* const*, const VT* const*, int,        \
                                     const double*, double, const VT*, VT*, uint64_t, double*,  \
-                                    hipStream_t);                                             \
+                                    hipStream_t, bool);                                        \
   template int vec_gs_update<VT>(sbd_ctx*, const VT* const*, int, const double*, VT*, double,  \
                                  uint64_t, double*, hipStream_t);                             \
   template int vec_lincomb<VT>(sbd_ctx*, const VT* const*, int, const double*, double, VT*,    \
