@@ -1,5 +1,7 @@
 """Block Davidson for several roots (sbd_diag_roots) against dense
-diagonalisation of the oracle's Hamiltonian on small FCI-sized systems.
+diagonalisation of the oracle's Hamiltonian on small FCI-sized systems, and
+on an open-shell norb-48 basis too large for that against the lowest
+eigenvalues ARPACK finds on the oracle's sigma (tests/golden/roots.json).
 
 The dense matrix is built column by column from the oracle's sigma (the CPU
 restatement of the reference's mult, itself pinned against second
@@ -8,8 +10,13 @@ eigenvalues.  Tolerances: energies 1e-9 Ha (FP64 block, and the FP32 block
 phase followed by the FP64 phase), Ritz vectors orthonormal to 1e-9 and
 eigenvectors of H to the solver's residual.
 """
+import json
+import os
+
 import numpy as np
 import pytest
+
+from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
@@ -84,3 +91,42 @@ def test_block_davidson_mixed_precision(torch):
     assert np.abs(E32 - E64).max() < 1e-9, (E32, E64)
     assert np.abs(E64 - ev[:4]).max() < 1e-9
     m.close()
+
+
+def test_block_davidson_open_shell_roots(torch):
+    """The 4 lowest states of an open-shell basis of the kind the multi-root
+    requests carry: norb 48, 12 + 10 electrons, sci48open's generator with
+    300 x 250 sampled strings plus the Hartree-Fock singles (461,160
+    determinants).  FP64 block Davidson, and the FP32 block phase followed by
+    the FP64 finish, from seeded random starts (init 1) with a 40-vector
+    subspace to residual 1e-6: each root within 1e-9 Ha of the one ARPACK finds
+    on the oracle's sigma (tests/golden/roots.json, make_roots.py, eigenpair
+    residuals below 1e-10; the Rayleigh-quotient error bound r^2 / gap is
+    7e-12 Ha here, smallest gap 0.147 Ha).  Convergence is slow on this basis
+    (about 400 steps to 1e-7 with 40 vectors, over 800 with 24).  Unit vectors
+    on the lowest diagonal elements (init 0) are not used: on this basis they
+    span an invariant subspace of H without the ground state (the solve
+    converges there, to states 80-210 Ha higher)."""
+    import sbd_amd
+    from sbd_amd import synth
+    gold = json.load(open(os.path.join(GOLDEN, "roots.json")))["sci48openhf"]
+    say("open-shell 4 roots, norb 48")
+    w = synth.make_workload(gold["workload"], gold["scale"])
+    m = sbd_amd.MultTPB(w["norb"], w["a"], w["b"], I0=w["I0"], h=w["h"], eri=w["eri"])
+    assert m.N == gold["N"]
+    ref = np.array(gold["roots"])
+    R = gold["nroots"]
+    opt64 = sbd_amd.SBD(max_it=600, max_nb=40, eps=1e-6, init=1, seed=5)
+    opt32 = sbd_amd.SBD(max_it=600, max_nb=40, eps=1e-6, init=1, seed=5, precision=1,
+                        fp32_eps=1e-2)
+    E64, res64, r64 = m.diag_roots(opt64, R)
+    say(f"fp64: {r64['steps']} steps, max |E - ARPACK| {np.abs(E64 - ref).max():.2e}")
+    E32, res32, r32 = m.diag_roots(opt32, R)
+    say(f"mixed: {r32['steps_fp32']} FP32 + {r32['steps'] - r32['steps_fp32']} FP64 steps, "
+        f"max |E - ARPACK| {np.abs(E32 - ref).max():.2e}")
+    m.close()
+    assert r64["converged"] == 1 and r32["converged"] == 1, (r64, r32)
+    assert np.all(res64 < 1e-6) and np.all(res32 < 1e-6)
+    assert r32["steps_fp32"] > 0
+    assert np.abs(E64 - ref).max() < 1e-9, (E64, ref)
+    assert np.abs(E32 - ref).max() < 1e-9, (E32, ref)
