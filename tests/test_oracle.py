@@ -338,6 +338,22 @@ def test_golden_energies_consistent(O):
         assert abs(d["energy"] - d["energy_eigsh"]) < 1e-9, key
 
 
+def test_golden_roots_consistent():
+    """The committed multi-root goldens (ARPACK on the oracle's sigma,
+    tests/golden/make_roots.py): ascending, each eigenpair's residual below
+    1e-9, the lowest below every diagonal element, and the workload is the one
+    the pinned generator makes (its size)."""
+    from sbd_amd import synth
+    g = json.load(open(os.path.join(GOLDEN, "roots.json")))
+    for key, d in g.items():
+        r = d["roots"]
+        assert len(r) == d["nroots"] and r == sorted(r)
+        assert max(d["residuals"]) < 1e-9, key
+        assert r[0] < d["hii_min"]
+        w = synth.make_workload(d["workload"], d["scale"])
+        assert len(w["a"]) * len(w["b"]) == d["N"]
+
+
 @pytest.mark.parametrize("L,na,nb,seed,sub", [(4, 2, 2, 3, 0), (5, 2, 2, 9, 0), (5, 3, 1, 4, 0),
                                               (6, 3, 2, 5, 12)])
 def test_oracle_correlation_matches_second_quantisation(O, L, na, nb, seed, sub):
