@@ -289,7 +289,10 @@ int sbd_save_matrix_form_wf(const char* filename, int norb, const uint64_t* a, s
  * 1..13; max_nb is the largest subspace (raised to 3 nroots, at most 40), max_it the
  * number of block iterations, eps the residual every root must reach; init 0
  * starts from the determinants with the lowest diagonal, init 1 from seeded
- * random vectors (seed, seed + 1, ...).  precision = 1 runs the FP32 block
+ * random vectors (seed, seed + 1, ...).  Where H splits into blocks that do
+ * not couple, the init 0 start finds the lowest states of the blocks its
+ * determinants lie in only (on the norb-48 open-shell test basis it misses the
+ * ground state); init 1 reaches every block.  precision = 1 runs the FP32 block
  * phase to fp32_eps first.  energies[nroots] = <W_r, H W_r> (fresh FP64
  * sigma), residuals[nroots] (optional), W_out (optional, host, nroots * N,
  * root-major).  Returns SBD_OK also when max_it ends the iterations first
