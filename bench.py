@@ -586,33 +586,44 @@ def main():
     }
 
     if not args.no_davidson:
-        res = {}
-        for label, prec in (("fp64", 0), ("mixed", 1)):
-            opt = sbd_amd.SBD(max_it=args.dav_restarts, max_nb=10, eps=1e-7, precision=prec)
-            t0 = time.time()
-            E, _, rep, hist = m.diag(opt, history=True)
-            res[label] = {"energy": E, "steps": rep["steps"], "steps_fp32": rep["steps_fp32"],
-                          "converged": rep["converged"], "residual": rep["residual"],
-                          "t_davidson": rep["t_davidson"], "wall": time.time() - t0,
-                          "hist": hist}
-            log(f"davidson {label}: E={E:.10f} steps={rep['steps']} "
-                f"t={rep['t_davidson']:.2f}s conv={rep['converged']}")
-        Eref = res["fp64"]["energy"]
-        dav = {}
-        for label, r in res.items():
-            tte = None
-            for E0, _, t, p in r["hist"]:
+        # each solve runs twice (the same iterates and energy both times); the
+        # faster run is reported and both times are listed: a one-off slowdown
+        # of a few seconds on the box then cannot decide the ratio
+        def tte_of(hist, Eref):
+            for E0, _, t, p in hist:
                 if p == 64 and abs(E0 - Eref) <= 1e-6:
-                    tte = t
-                    break
+                    return t
+            return None
+
+        runs = {}
+        for rep_i in range(2):
+            for label, prec in (("fp64", 0), ("mixed", 1)):
+                opt = sbd_amd.SBD(max_it=args.dav_restarts, max_nb=10, eps=1e-7, precision=prec)
+                E, _, rep, hist = m.diag(opt, history=True)
+                runs.setdefault(label, []).append(
+                    {"energy": E, "steps": rep["steps"], "steps_fp32": rep["steps_fp32"],
+                     "converged": rep["converged"], "residual": rep["residual"],
+                     "t_davidson": rep["t_davidson"], "hist": hist})
+                log(f"davidson {label} run {rep_i}: E={E:.10f} steps={rep['steps']} "
+                    f"t={rep['t_davidson']:.2f}s conv={rep['converged']}")
+        Eref = runs["fp64"][0]["energy"]
+        dav = {}
+        for label, rs in runs.items():
+            ttes = [tte_of(r["hist"], Eref) for r in rs]
+            best = min(range(len(rs)), key=lambda i: (ttes[i] is None, ttes[i] or 0.0))
+            r, tte = rs[best], ttes[best]
             dav[label] = {"time_to_energy_s": None if tte is None else round(tte, 4),
+                          "time_to_energy_runs_s": [None if x is None else round(x, 4)
+                                                    for x in ttes],
                           "energy": r["energy"], "abs_err_vs_fp64": abs(r["energy"] - Eref),
+                          "energies_equal_across_runs": len({x["energy"] for x in rs}) == 1,
                           "steps": r["steps"], "steps_fp32": r["steps_fp32"],
                           "converged": bool(r["converged"]), "residual": r["residual"],
                           "t_davidson_s": round(r["t_davidson"], 4)}
         t64, tmx = dav["fp64"]["time_to_energy_s"], dav["mixed"]["time_to_energy_s"]
         dav["speedup_mixed_vs_fp64"] = (round(t64 / tmx, 3) if t64 and tmx else None)
         dav["eps"] = 1e-7
+        dav["runs"] = "each solve twice, the faster reported"
         out["davidson"] = dav
 
     if not args.no_cpu:
