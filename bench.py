@@ -157,6 +157,8 @@ MOLECULES = [
      "data/c2h2/README.md:7"),
     ("h2o_1em6", "h2o/fcidump.txt.gz", "h2o/h2o-1em6-alpha.txt.gz", -76.24377593,
      "data/h2o/README.md:11"),
+    ("n2_1em7", "n2/fcidump.txt.gz", "n2/1em7-alpha.txt.gz", -109.04874198,
+     "data/n2/README.md:14"),
 ]
 
 

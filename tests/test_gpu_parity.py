@@ -98,6 +98,11 @@ CASES = [
     ("n2_1em3", lambda: make_pair_fcidump("n2", "1em3-alpha.txt.gz")),
     # the reference's Fe4S4 cluster: norb 36, two-word determinants
     ("fe4s4", lambda: make_pair_fcidump("fe4s4", "alpha.txt.gz")),
+    # rows of 19,000 beta strings (19 per thread of the dense alpha x beta row
+    # pass: its 24-column form, one FP64 buffer, two FP32 buffers) under three
+    # alpha strings with two single excitations each
+    ("wide_rows_ab24", lambda: make_pair(custom_case(22, [0b011, 0b101, 0b110],
+                                                     _all(22, 5)[:19000], 2224, 44))),
 ]
 
 
