@@ -609,6 +609,18 @@ def main():
                 log(f"davidson {label} run {rep_i}: E={E:.10f} steps={rep['steps']} "
                     f"t={rep['t_davidson']:.2f}s conv={rep['converged']}")
         Eref = runs["fp64"][0]["energy"]
+
+        def step_ms(hist):
+            # wall time between consecutive Ritz values: min / median / max and
+            # the step of the slowest (tells a uniformly slow run from a stall)
+            import numpy as np
+            t = np.asarray([row[2] for row in hist])
+            if t.size < 2:
+                return None
+            dt = np.diff(t) * 1e3
+            return {"min": round(float(dt.min()), 2), "median": round(float(np.median(dt)), 2),
+                    "max": round(float(dt.max()), 2), "argmax": int(dt.argmax()) + 1}
+
         dav = {}
         for label, rs in runs.items():
             ttes = [tte_of(r["hist"], Eref) for r in rs]
@@ -617,6 +629,7 @@ def main():
             dav[label] = {"time_to_energy_s": None if tte is None else round(tte, 4),
                           "time_to_energy_runs_s": [None if x is None else round(x, 4)
                                                     for x in ttes],
+                          "step_ms_runs": [step_ms(x["hist"]) for x in rs],
                           "energy": r["energy"], "abs_err_vs_fp64": abs(r["energy"] - Eref),
                           "energies_equal_across_runs": len({x["energy"] for x in rs}) == 1,
                           "steps": r["steps"], "steps_fp32": r["steps_fp32"],
