@@ -156,9 +156,14 @@ MOLECULES = [
     ("c2h2_100k", "c2h2/fcidump.txt.gz", "c2h2/alpha-100k.txt.gz", -77.11337996,
      "data/c2h2/README.md:7"),
     ("h2o_1em6", "h2o/fcidump.txt.gz", "h2o/h2o-1em6-alpha.txt.gz", -76.24377593,
-     "data/h2o/README.md:11"),
+     "data/h2o/README.md:9"),
     ("n2_1em7", "n2/fcidump.txt.gz", "n2/1em7-alpha.txt.gz", -109.04874198,
      "data/n2/README.md:14"),
+]
+# larger inputs, run with --large-molecules (minutes each)
+MOLECULES_LARGE = [
+    ("h2o_1em7", "h2o/fcidump.txt.gz", "h2o/h2o-1em7-alpha.txt.gz", -76.24377678,
+     "data/h2o/README.md:10"),
 ]
 
 
@@ -227,7 +232,8 @@ def dense_section(args):
     if args.no_molecules:
         return out
     with tempfile.TemporaryDirectory() as tmp:
-        for name, fd_rel, a_rel, E_held, cite in MOLECULES:
+        for name, fd_rel, a_rel, E_held, cite in MOLECULES + (
+                MOLECULES_LARGE if args.large_molecules else []):
             t0 = time.time()
             fd = sbd_amd.LoadFCIDump(_plain_copy(tmp, fd_rel))
             a = sbd_amd.LoadAlphaDets(_plain_copy(tmp, a_rel), fd.norb)
@@ -239,7 +245,25 @@ def dense_section(args):
             T = torch.randn(m.N, dtype=torch.float64, device="cuda", generator=g)
             ms = time_sigma(m, hii, T, 2, warmup=1)
             del T, hii
-            E, _, rep, hist = m.diag(sbd_amd.SBD(max_it=60, max_nb=10, eps=1e-6), history=True)
+            log(f"dense {name}: N={m.N:.3e} sigma {ms:.2f} ms")
+            if (name, fd_rel, a_rel, E_held, cite) in MOLECULES_LARGE:
+                # one restart cycle (10 steps) per call, each started from the
+                # previous cycle's Ritz vector as the solver's own restart is,
+                # so that a line is logged after each cycle
+                W, steps, t_dav = None, 0, 0.0
+                for cyc in range(60):
+                    E, W, rep = m.diag(sbd_amd.SBD(max_it=1, max_nb=10, eps=1e-6),
+                                       want_vector=True, W_init=W)
+                    steps += rep["steps"]
+                    t_dav += rep["t_davidson"]
+                    log(f"dense {name}: cycle {cyc}: E={E:.10f} residual {rep['residual']:.2e} "
+                        f"steps {steps} t {t_dav:.1f}s")
+                    if rep["converged"]:
+                        break
+                rep = dict(rep, steps=steps, t_davidson=t_dav)
+                del W
+            else:
+                E, _, rep = m.diag(sbd_amd.SBD(max_it=60, max_nb=10, eps=1e-6))
             out[name] = {"determinants": m.N, "norb": fd.norb, "strings": len(a),
                          "terms_per_det": round(terms / m.N, 1), "sigma_ms": round(ms, 3),
                          "sigma_gdet_s": round(m.N / ms * 1e-6, 3),
@@ -478,6 +502,9 @@ def main():
                     help="longest host Davidson (oracle, FCI norb 12) the dense section runs")
     ap.add_argument("--no-molecules", action="store_true",
                     help="dense section without the reference's molecule inputs")
+    ap.add_argument("--large-molecules", action="store_true",
+                    help="dense section also on the reference's H2O 1e-7 input (6.28e8 "
+                         "determinants, minutes)")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--prec-max-it", type=int, default=80,
                     help="Davidson restarts of the precision section (subspace 10, or what fits)")

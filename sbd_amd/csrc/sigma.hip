@@ -1303,6 +1303,8 @@ __global__ void k_ab_sell(uint32_t nb, const uint32_t* __restrict__ off,
 // same operations, so the results are the same bits.
 constexpr int kElemBlock = 256;
 constexpr double kSparseTerms = 2.0;  // off-diagonal pairs per alpha string + per beta string
+// the same sum at which rows too wide for LDS take the row kernel (L2 gathers)
+constexpr double kDenseRowTerms = 64.0;
 constexpr int kElemCols = 4;
 constexpr int kElemTile = kElemBlock * kElemCols;
 
@@ -1508,20 +1510,23 @@ static int sigma_rows_for(const sbd_ctx* ctx, size_t elem) {
 // Which kernel a context's sigma takes.  One rule serves single launches and
 // the serving engine's batches, so a solve computes the same way alone and in
 // a batch: the per-element kernel for sparse bases (fewer than kSparseTerms
-// off-diagonal pairs per alpha string plus per beta string), else the split
-// form or the row kernel (sigma_use_split); sbd_set_sigma_kernel can force
-// any of them.
+// off-diagonal pairs per alpha string plus per beta string; fewer than
+// kDenseRowTerms when a row does not fit LDS), else the split form or the row
+// kernel (sigma_use_split); sbd_set_sigma_kernel can force any of them.
 static bool sigma_use_elems(const sbd_ctx* ctx, size_t elem) {
   if (ctx->sigma_kernel == 1 || ctx->sigma_kernel == 3 || ctx->sigma_kernel == 4) return false;
   if (ctx->sigma_kernel == 2) return true;
-  // rows that do not fit LDS: the row kernel would gather every beta
-  // excitation's T element from L2 (17x slower than this kernel on 60,480
-  // strings of 58 orbitals with the HF singles)
-  if (elem * ctx->nb > (size_t)ctx->lds_max) return true;
   const SpinTables& A = ctx->ta;
   const SpinTables& B = ctx->tb;
   const double terms = (double)(A.n_s + A.n_d) / (double)ctx->na +
                        (double)(B.n_s + B.n_d) / (double)ctx->nb;
+  // rows that do not fit LDS: the row kernel gathers every beta excitation's
+  // T element from L2.  On nearly sparse bases that is far slower than this
+  // kernel (17x on 60,480 strings of 58 orbitals with the HF singles, about
+  // 4 pairs per string); on dense ones far faster (the reference's H2O 1e-7,
+  // 25,062 strings, about 2,500 pairs per alpha plus beta string: 11.1 s
+  // against 71.3 s per FP64 sigma)
+  if (elem * ctx->nb > (size_t)ctx->lds_max) return terms < kDenseRowTerms;
   return terms < kSparseTerms;
 }
 

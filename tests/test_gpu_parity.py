@@ -103,6 +103,11 @@ CASES = [
     # alpha strings with two single excitations each
     ("wide_rows_ab24", lambda: make_pair(custom_case(22, [0b011, 0b101, 0b110],
                                                      _all(22, 5)[:19000], 2224, 44))),
+    # FP64 rows of 21,000 beta strings (168 KB) do not fit LDS: a dense basis
+    # takes the row kernel with L2 gathers (and the column-block dense pass),
+    # FP32 rows the 24-column row pass with one buffer
+    ("dense_rows_past_lds", lambda: make_pair(custom_case(22, [0b011, 0b101, 0b110],
+                                                          _all(22, 5)[:21000], 2226, 44))),
 ]
 
 

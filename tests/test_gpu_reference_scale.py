@@ -5,7 +5,7 @@ Reference-held energies (8-decimal roundings, so they pin a result to about
 5e-9 Ha; the bar is 6e-9):
 
   H2O cc-pVDZ, threshold 1e-4   1,543 strings   2.38e6 dets  -76.24295848  data/h2o/README.md:7
-  N2 6-31G,    threshold 1e-4   1,387 strings   1.92e6 dets -109.04835269  data/n2/README.md:9
+  N2 6-31G,    threshold 1e-4   1,387 strings   1.92e6 dets -109.04835269  data/n2/README.md:8
   H2O cc-pVDZ, threshold 1e-5   5,514 strings   3.04e7 dets  -76.24373504  data/h2o/README.md:8
   C2H2 cc-pVDZ, 30k TrimCI core 4,116 strings   1.69e7 dets  -77.10918904  data/c2h2/README.md:6
 
