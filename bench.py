@@ -164,6 +164,8 @@ MOLECULES = [
 MOLECULES_LARGE = [
     ("h2o_1em7", "h2o/fcidump.txt.gz", "h2o/h2o-1em7-alpha.txt.gz", -76.24377678,
      "data/h2o/README.md:10"),
+    ("n2_fci", "n2/fcidump.txt.gz", "n2/full-alpha.txt.gz", -109.04874199,
+     "data/n2/README.md:15"),
 ]
 
 
@@ -246,24 +248,21 @@ def dense_section(args):
             ms = time_sigma(m, hii, T, 2, warmup=1)
             del T, hii
             log(f"dense {name}: N={m.N:.3e} sigma {ms:.2f} ms")
-            if (name, fd_rel, a_rel, E_held, cite) in MOLECULES_LARGE:
-                # one restart cycle (10 steps) per call, each started from the
-                # previous cycle's Ritz vector as the solver's own restart is,
-                # so that a line is logged after each cycle
-                W, steps, t_dav = None, 0, 0.0
-                for cyc in range(60):
-                    E, W, rep = m.diag(sbd_amd.SBD(max_it=1, max_nb=10, eps=1e-6),
-                                       want_vector=True, W_init=W)
-                    steps += rep["steps"]
-                    t_dav += rep["t_davidson"]
-                    log(f"dense {name}: cycle {cyc}: E={E:.10f} residual {rep['residual']:.2e} "
-                        f"steps {steps} t {t_dav:.1f}s")
-                    if rep["converged"]:
-                        break
-                rep = dict(rep, steps=steps, t_davidson=t_dav)
-                del W
-            else:
-                E, _, rep = m.diag(sbd_amd.SBD(max_it=60, max_nb=10, eps=1e-6))
+            large = (name, fd_rel, a_rel, E_held, cite) in MOLECULES_LARGE
+            # the large solves print each Davidson step (the solver's verbose
+            # lines, sent to stderr: stdout carries the result line only)
+            saved = None
+            if large:
+                sys.stdout.flush()
+                saved = os.dup(1)
+                os.dup2(2, 1)
+            try:
+                E, _, rep = m.diag(sbd_amd.SBD(max_it=60, max_nb=10, eps=1e-6,
+                                               verbose=1 if large else 0))
+            finally:
+                if saved is not None:
+                    os.dup2(saved, 1)
+                    os.close(saved)
             out[name] = {"determinants": m.N, "norb": fd.norb, "strings": len(a),
                          "terms_per_det": round(terms / m.N, 1), "sigma_ms": round(ms, 3),
                          "sigma_gdet_s": round(m.N / ms * 1e-6, 3),
@@ -503,8 +502,8 @@ def main():
     ap.add_argument("--no-molecules", action="store_true",
                     help="dense section without the reference's molecule inputs")
     ap.add_argument("--large-molecules", action="store_true",
-                    help="dense section also on the reference's H2O 1e-7 input (6.28e8 "
-                         "determinants, minutes)")
+                    help="dense section also on the reference's H2O 1e-7 and N2 full-CI "
+                         "inputs (6.28e8 and 1.01e9 determinants, minutes each)")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--prec-max-it", type=int, default=80,
                     help="Davidson restarts of the precision section (subspace 10, or what fits)")
