@@ -166,6 +166,8 @@ MOLECULES_LARGE = [
      "data/h2o/README.md:10"),
     ("n2_fci", "n2/fcidump.txt.gz", "n2/full-alpha.txt.gz", -109.04874199,
      "data/n2/README.md:15"),
+    ("h2o_1em8", "h2o/fcidump.txt.gz", "h2o/h2o-1em8-alpha.txt.gz", -76.24377680,
+     "data/h2o/README.md:11"),
 ]
 
 
@@ -231,11 +233,17 @@ def dense_section(args):
     out["fci12"] = fci
     m.close()
     del T, hii
-    if args.no_molecules:
+    large_names = [x for x in args.large_molecules.split(",") if x]
+    known = [x[0] for x in MOLECULES_LARGE]
+    for x in large_names:
+        if x not in known:
+            raise SystemExit(f"--large-molecules: unknown input {x} (one of {', '.join(known)})")
+    mols = ([] if args.no_molecules else MOLECULES) + [x for x in MOLECULES_LARGE
+                                                       if x[0] in large_names]
+    if not mols:
         return out
     with tempfile.TemporaryDirectory() as tmp:
-        for name, fd_rel, a_rel, E_held, cite in MOLECULES + (
-                MOLECULES_LARGE if args.large_molecules else []):
+        for name, fd_rel, a_rel, E_held, cite in mols:
             t0 = time.time()
             fd = sbd_amd.LoadFCIDump(_plain_copy(tmp, fd_rel))
             a = sbd_amd.LoadAlphaDets(_plain_copy(tmp, a_rel), fd.norb)
@@ -247,6 +255,7 @@ def dense_section(args):
             T = torch.randn(m.N, dtype=torch.float64, device="cuda", generator=g)
             ms = time_sigma(m, hii, T, 2, warmup=1)
             del T, hii
+            torch.cuda.empty_cache()  # the solve's vectors get the memory torch held
             log(f"dense {name}: N={m.N:.3e} sigma {ms:.2f} ms")
             large = (name, fd_rel, a_rel, E_held, cite) in MOLECULES_LARGE
             # the large solves print each Davidson step (the solver's verbose
@@ -501,9 +510,10 @@ def main():
                     help="longest host Davidson (oracle, FCI norb 12) the dense section runs")
     ap.add_argument("--no-molecules", action="store_true",
                     help="dense section without the reference's molecule inputs")
-    ap.add_argument("--large-molecules", action="store_true",
-                    help="dense section also on the reference's H2O 1e-7 and N2 full-CI "
-                         "inputs (6.28e8 and 1.01e9 determinants, minutes each)")
+    ap.add_argument("--large-molecules", nargs="?", const="h2o_1em7,n2_fci", default="",
+                    help="dense section also on these of the reference's larger inputs, "
+                         "comma-separated: h2o_1em7 (6.28e8 determinants), n2_fci (1.01e9), "
+                         "h2o_1em8 (1.52e9; about 11 minutes); no list: the first two")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--prec-max-it", type=int, default=80,
                     help="Davidson restarts of the precision section (subspace 10, or what fits)")
