@@ -117,8 +117,8 @@ def terms_per_sigma(m, counts):
 
 
 def time_sigma(m, hii, T, steps, warmup=2):
-    """Average milliseconds of one FP64 sigma launch (HIP events on the stream
-    the launches go to)."""
+    """Average milliseconds of one sigma launch in T's precision (HIP events on
+    the stream the launches go to)."""
     import torch
     Wb = torch.empty_like(T)
     for _ in range(warmup):
@@ -254,9 +254,15 @@ def dense_section(args):
             hii = m.hdiag()
             T = torch.randn(m.N, dtype=torch.float64, device="cuda", generator=g)
             ms = time_sigma(m, hii, T, 2, warmup=1)
-            del T, hii
+            del hii
+            T32 = T.float()
+            del T
+            torch.cuda.empty_cache()
+            h32 = m.hdiag(torch.float32)
+            ms32 = time_sigma(m, h32, T32, 2, warmup=1)
+            del T32, h32
             torch.cuda.empty_cache()  # the solve's vectors get the memory torch held
-            log(f"dense {name}: N={m.N:.3e} sigma {ms:.2f} ms")
+            log(f"dense {name}: N={m.N:.3e} sigma {ms:.2f} ms (FP32 {ms32:.2f} ms)")
             large = (name, fd_rel, a_rel, E_held, cite) in MOLECULES_LARGE
             # the large solves print each Davidson step (the solver's verbose
             # lines, sent to stderr: stdout carries the result line only)
@@ -274,6 +280,7 @@ def dense_section(args):
                     os.close(saved)
             out[name] = {"determinants": m.N, "norb": fd.norb, "strings": len(a),
                          "terms_per_det": round(terms / m.N, 1), "sigma_ms": round(ms, 3),
+                         "sigma_fp32_ms": round(ms32, 3),
                          "sigma_gdet_s": round(m.N / ms * 1e-6, 3),
                          "sigma_gterms_s": round(terms / ms * 1e-6, 2),
                          "energy": E, "energy_held": E_held, "held_at": cite,

@@ -1140,11 +1140,11 @@ __global__ __launch_bounds__(kAbCols) void k_sigma_ab(SigmaArgs<VT> p, uint32_t 
   }
 }
 
-// Row-wise form of the dense pass (bases of up to kAbRowBlock * 24 beta
+// Row-wise form of the dense pass (bases of up to kAbRowBlock * 40 beta
 // strings whose row and integral slice fit LDS): one 1024-thread workgroup per
 // bra row (grid-stride over rows), thread t owning columns t + 1024 j (j < C)
 // with their sums kept across all of the row's alpha singles (in registers up
-// to 8 columns, in private memory at 16 and 24).  For each alpha single the ket row
+// to 8 columns, in private memory from 16 on).  For each alpha single the ket row
 // T[ja, :] and the integral slice are staged in LDS; with two buffers (when
 // they fit) the next ones are loaded into registers while the current ones are
 // used and stored after, one barrier per alpha single.  The beta records come
@@ -1234,7 +1234,7 @@ __global__ __launch_bounds__(kAbRowBlock) void k_sigma_ab_rows(SigmaArgs<VT> p,
       const uint32_t sA = parity64(A & bits_range(lo + 1, hi));
       const VT* tr = trow(cur);
       const double* vs = vslc(cur);
-      // unrolled up to 8 columns per thread; at 16 and 24 the unrolled copies
+      // unrolled up to 8 columns per thread; from 16 on the unrolled copies
       // spill (412 B/lane FP64 at 16), so the loop stays rolled and the sums are
       // indexed by the uniform column counter (H2O 1e-6: 1.62 -> 1.56 s per sigma)
 #pragma unroll(C > 8 ? 1 : C)
@@ -1592,8 +1592,15 @@ static int sigma_ab_rows_cfg(sbd_ctx* ctx, size_t elem, int* C, bool* dbuf, size
                              hipStream_t st) {
   *C = 0;
   const uint64_t per = (ctx->nb + kAbRowBlock - 1) / kAbRowBlock;
-  if (per > 24) return SBD_OK;  // wider rows take the column-block form
-  *C = per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : per <= 8 ? 8 : per <= 16 ? 16 : 24;
+  if (per > 40) return SBD_OK;  // wider rows take the column-block form
+  *C = per <= 1    ? 1
+       : per <= 2  ? 2
+       : per <= 4  ? 4
+       : per <= 8  ? 8
+       : per <= 16 ? 16
+       : per <= 24 ? 24
+       : per <= 32 ? 32
+                   : 40;
   const size_t row_bytes = (elem * ctx->nb + 15) & ~(size_t)15;
   const size_t buf = row_bytes + sizeof(double) * ctx->norb * ctx->norb;
   *dbuf = 2 * buf <= (size_t)ctx->lds_max;
@@ -1909,7 +1916,9 @@ static int launch_rows(sbd_ctx* ctx, const VT* hii, const VT* T, VT* W, int accu
         case 4: SBD_AB_ROWS(4); break;
         case 8: SBD_AB_ROWS(8); break;
         case 16: SBD_AB_ROWS(16); break;
-        default: SBD_AB_ROWS(24); break;
+        case 24: SBD_AB_ROWS(24); break;
+        case 32: SBD_AB_ROWS(32); break;
+        default: SBD_AB_ROWS(40); break;
       }
 #undef SBD_AB_ROWS
     } else {
@@ -2030,7 +2039,9 @@ int sigma_set_lds_limit(sbd_ctx* ctx) {
       (const void*)k_sigma_ab_rows<VT, 1, false>, (const void*)k_sigma_ab_rows<VT, 2, false>,     \
       (const void*)k_sigma_ab_rows<VT, 4, false>, (const void*)k_sigma_ab_rows<VT, 8, false>, \
       (const void*)k_sigma_ab_rows<VT, 16, true>, (const void*)k_sigma_ab_rows<VT, 16, false>, \
-      (const void*)k_sigma_ab_rows<VT, 24, true>, (const void*)k_sigma_ab_rows<VT, 24, false>
+      (const void*)k_sigma_ab_rows<VT, 24, true>, (const void*)k_sigma_ab_rows<VT, 24, false>, \
+      (const void*)k_sigma_ab_rows<VT, 32, true>, (const void*)k_sigma_ab_rows<VT, 32, false>, \
+      (const void*)k_sigma_ab_rows<VT, 40, true>, (const void*)k_sigma_ab_rows<VT, 40, false>
       SBD_AB_FNS(double), SBD_AB_FNS(float)
 #undef SBD_AB_FNS
   };

@@ -108,6 +108,10 @@ CASES = [
     # FP32 rows the 24-column row pass with one buffer
     ("dense_rows_past_lds", lambda: make_pair(custom_case(22, [0b011, 0b101, 0b110],
                                                           _all(22, 5)[:21000], 2226, 44))),
+    # 26,000 beta strings: FP32 rows (104 KB) take the 32-column row pass, FP64
+    # rows (208 KB) the column-block pass
+    ("dense_rows_c32", lambda: make_pair(custom_case(22, [0b011, 0b101, 0b110],
+                                                     _all(22, 5)[:26000], 2228, 44))),
 ]
 
 
