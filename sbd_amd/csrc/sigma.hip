@@ -1152,7 +1152,9 @@ __global__ __launch_bounds__(kAbCols) void k_sigma_ab(SigmaArgs<VT> p, uint32_t 
 // wavefront's record loads are coalesced; every gather of the inner loop is
 // from LDS.  Same terms, same order, same operations as k_sigma_ab.
 constexpr int kAbRowBlock = 1024;
-constexpr int kAbRowBatch = 8;
+// records whose LDS operands are fetched together (4 / 8 / 12 / 16 measured on the
+// reference's H2O 1e-6: 1,692 / 1,563 / 1,581 / 1,554 ms per FP64 sigma)
+constexpr int kAbRowBatch = 16;
 
 template <typename VT, int C, bool kDbuf>
 __global__ __launch_bounds__(kAbRowBlock) void k_sigma_ab_rows(SigmaArgs<VT> p,
