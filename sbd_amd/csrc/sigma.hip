@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_bd_batch(const SigmaJob<VT>
 #define SBD_AB_COLS 256
 #endif
 #ifndef SBD_AB_BATCH
-#define SBD_AB_BATCH 8
+#define SBD_AB_BATCH 16  // 4 / 8 / 16 on the reference's H2O 1e-7 FP64: 12.86 / 11.38 / 11.25 s
 #endif
 constexpr int kAbCols = SBD_AB_COLS;    // columns (threads) per workgroup
 constexpr int kAbKetBits = 19;          // beta strings per basis for the packed records
