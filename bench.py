@@ -172,7 +172,7 @@ MOLECULES_LARGE = [
 
 
 def dense_section(args):
-    """Config 1 (FCI norb 12) on the GPU and on the host, and the reference's
+    """The dense workload (FCI norb 12) on the GPU and on the host, and the reference's
     larger molecule inputs against their held energies."""
     import numpy as np
     import torch
@@ -295,8 +295,9 @@ def dense_section(args):
 
 
 def precision_section(args):
-    """Config 3 at its size: plain FP64 Davidson against FP32 Davidson + FP64
-    sweep on 3.66e9 determinants, both from the HF determinant (init 0)."""
+    """The precision workload at its size: plain FP64 Davidson against FP32
+    Davidson + FP64 sweep on 3.66e9 determinants, both from the HF determinant
+    (init 0)."""
     import sbd_amd
     from sbd_amd import synth
     t0 = time.time()
@@ -332,7 +333,7 @@ def precision_section(args):
 
 
 def multiroot_section(args):
-    """Config 5, reduced: concurrent block-Davidson requests for the 4 lowest
+    """The multi-root workload, strings scaled down: concurrent block-Davidson requests for the 4 lowest
     roots (sampled strings as configured, started on the lowest diagonal
     elements), served in
     FP32 (FP64 Ritz and Gram-Schmidt, FP64 finish) and again in FP64; per-root
