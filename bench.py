@@ -172,8 +172,8 @@ MOLECULES_LARGE = [
 
 
 def dense_section(args):
-    """The dense workload (FCI norb 12) on the GPU and on the host, and the reference's
-    larger molecule inputs against their held energies."""
+    """The dense workload (FCI norb 12) on the GPU and on the host, and the
+    reference's larger molecule inputs against their held energies."""
     import numpy as np
     import torch
     import sbd_amd
@@ -333,8 +333,8 @@ def precision_section(args):
 
 
 def multiroot_section(args):
-    """The multi-root workload, strings scaled down: concurrent block-Davidson requests for the 4 lowest
-    roots (sampled strings as configured, started on the lowest diagonal
+    """The multi-root workload, strings scaled down: concurrent block-Davidson
+    requests for the 4 lowest roots (sampled strings as configured, started on the lowest diagonal
     elements), served in
     FP32 (FP64 Ritz and Gram-Schmidt, FP64 finish) and again in FP64; per-root
     energy error of the FP32 results against the FP64 ones."""
